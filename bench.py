@@ -1,0 +1,173 @@
+"""Benchmark of the plain FP64 ILES time step on one MI355X.
+
+Workload (default): single triply periodic block of 256^3 cells (257^3
+stored points), Taylor-Green vortex at Mach 0.1, Re = 1600, seeded uniform
+velocity perturbation of amplitude 1e-3, viscous, RK4 with cfl = 0.8, FP64.
+Synthetic inputs are generated on the host and uploaded before timing; the
+timed region is K whole time steps (filter phase + 4 RK stages each) with the
+state resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--n 256] [--precision fp64|mixed]
+
+Prints one JSON line.  value = nominal cells x 4 RK stages x K / seconds.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s per RK step + % HBM peak; mixed vs plain FP64 at stated error"
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s (AMD public figure)
+STEP_BYTES_MODEL = 5048.0  # compulsory FP64 bytes per stored point per viscous step
+
+# Algorithmic HBM bytes per stored point of one launch of each kernel group
+# (reads + writes of FP64 fields the kernel must touch):
+#   filter sweep (5 variables, one direction): read 5 + write 5 fields
+#   gradient sweep (u,v,w,T along one direction): read q (5) + write 4
+#   stress assembly: read q 5, 12 gradients, 9 metrics + Jacobian, write 9
+#   flux sweep: read q 5, stresses 9, 3 metrics; write de 5 (xi) or
+#               read+write de 5 (eta, zeta) -> mean over the three launches
+#   update: read qo 5, de 5, Jacobian; write qa 5
+SECTION_BYTES = {"filter": 80.0, "grad": 72.0, "stress": 288.0,
+                 "flux": (176.0 + 216.0 + 216.0) / 3.0, "update": 128.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="replay whole steps as a hipGraph (section timers then stay empty)")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same as --n)")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(section, n):
+    """HBM bytes per launch of `section` from the committed rocprofv3 PMC
+    summary (profiles/traffic_tgv<n>.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_tgv{n}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| " + section + " |"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(n, steps):
+    """Times the CPU oracle (C restatement of the reference, OpenMP over
+    pencils) on the same TGV workload, on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=False)
+    from helpers import oracle_for, tgv_case
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    blocks, grids, states, params = tgv_case(n, pert=1e-3)
+    o = oracle_for(blocks, grids, states, params, threads=threads)
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    cells = n ** 3
+    return {"value": cells * 4 * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} time step(s) of the {n}^3 TGV (4 RK stages each) on {threads} OpenMP threads, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the per-block step does not shard "
+              "across GPUs in this build.", file=sys.stderr)
+        sys.exit(2)
+    from canard_amd.solver import CND_FP64, CND_MIXED, Case
+
+    n = a.n
+    from canard_amd import cases
+    blocks = [cases.periodic_block(n)]
+    xyz = cases.gridtoy(n, n, n, doml0=0.0, doml1=2 * np.pi, domh=np.pi, span=2 * np.pi)
+    grids = [xyz]
+    states = [cases.tgv_state(xyz, (n + 1, n + 1, n + 1), mach=0.1, pert=1e-3, seed=20261202)]
+    params = dict(fltk=0.49, fltrbc=0.9, reoo=1600.0 / 0.1, tempoo=300.0, cfl=0.8, viscous=1)
+    prec = CND_FP64 if a.precision == "fp64" else CND_MIXED
+    print(f"[bench] {n}^3 TGV, {a.precision}: set-up", file=sys.stderr, flush=True)
+    c = Case(blocks, params, precision=prec)
+    c.set_grids(grids)
+    c.set_states(states)
+    c.set_clock()
+    del grids, states
+    c.use_graph(bool(a.graph))
+    c.step(a.warmup)
+    c.synchronize()
+    print(f"[bench] warm-up done, timing {a.steps} steps", file=sys.stderr, flush=True)
+    c.profile(True)
+    t0 = time.perf_counter()
+    c.step(a.steps)
+    c.synchronize()
+    dt = time.perf_counter() - t0
+    sections = c.profile_read()
+    c.profile(False)
+    clock = c.clock()
+    q = c.state(0)
+    finite = bool(np.all(np.isfinite(q)))
+    points, cells = c.points, c.cells
+    c.close()
+
+    value = cells * 4 * a.steps / dt
+    # dominant kernel group by measured time
+    timed = {k: v for k, v in sections.items() if k in SECTION_BYTES and v[1] > 0}
+    roof = None
+    if timed:
+        dom = max(timed, key=lambda k: timed[k][0])
+        ms, launches = timed[dom]
+        avg_s = ms / 1e3 / launches
+        bytes_launch = SECTION_BYTES[dom] * points
+        ach = bytes_launch / avg_s
+        roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": traffic_from_profiles(dom, n),
+                "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_s * 1e3}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == CND_FP64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"taylor_green_{n}^3_periodic_viscous_rk4", "cells": cells,
+                   "stored_points": points, "mach": 0.1, "re": 1600.0, "perturbation": 1e-3,
+                   "precision": a.precision, "graph": bool(a.graph)},
+        "hbm_pct_step_model": 100.0 * STEP_BYTES_MODEL * points * a.steps / dt / HBM_PEAK,
+        "roofline": roof,
+        "sections_ms_per_step": {k: v[0] / a.steps for k, v in sections.items()},
+        "final_clock": clock,
+        "finite": finite,
+    }
+    print(f"[bench] GPU {dt:.2f} s; CPU baseline", file=sys.stderr, flush=True)
+    if a.cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_n or n, a.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,196 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on
+the same inputs.  Tolerances: single sweeps and metrics 1e-13 relative to the
+field's max (the device follows the oracle's evaluation order; only math
+library ulps differ); whole steps 1e-12 per conserved variable (max-norm
+relative), the north-star parity target."""
+import numpy as np
+import pytest
+
+from canard_amd import cases
+from helpers import oracle_for, rel_err, tgv_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(blocks, grids, states, params, **kw):
+    from canard_amd.solver import Case
+    c = Case(blocks, params, **kw)
+    c.set_grids(grids)
+    c.set_states(states)
+    c.set_clock()
+    return c
+
+
+def two_block_x(n=(20, 16), let=16, lze=12, code=35):
+    """Blocks 0 and 1 joined on xi faces (BC code), periodic in x through each
+    other, periodic in y and z."""
+    b0 = dict(lxi=n[0], let=let, lze=lze, nbc=[[code, code], [45, 45], [45, 45]], mcd=[[1, 1], [0, 0], [0, 0]])
+    b1 = dict(lxi=n[1], let=let, lze=lze, nbc=[[code, code], [45, 45], [45, 45]], mcd=[[0, 0], [1, 1], [1, 1]])
+    h = 2 * np.pi / (n[0] + n[1])
+    g0 = cases.gridtoy(n[0], let, lze, doml0=0.0, doml1=n[0] * h, domh=np.pi, span=2 * np.pi)
+    g1 = cases.gridtoy(n[1], let, lze, doml0=-n[0] * h, doml1=(n[0] + n[1]) * h, domh=np.pi, span=2 * np.pi)
+    return [b0, b1], [g0, g1]
+
+
+SWEEP_CASES = {
+    "periodic16": lambda: ([cases.periodic_block(16)], [cases.gridtoy(16, 16, 16)]),
+    "mixed_bc": lambda: ([dict(lxi=24, let=20, lze=16, nbc=[[10, 10], [45, 45], [25, 20]],
+                               mcd=[[-1, -1], [0, 0], [-1, -1]])], [cases.gridtoy(24, 20, 16)]),
+    "two_block": lambda: two_block_x(),
+}
+
+
+@pytest.mark.parametrize("name", list(SWEEP_CASES))
+def test_single_sweeps_match_oracle(name):
+    blocks, grids = SWEEP_CASES[name]()
+    rng = np.random.default_rng(11)
+    npt = [(b["lxi"] + 1) * (b["let"] + 1) * (b["lze"] + 1) for b in blocks]
+    fields = [np.sin(g[0]) * np.cos(2 * g[1]) + 0.3 * rng.standard_normal(n) for g, n in zip(grids, npt)]
+    states = [cases.uniform_state(n) for n in npt]
+    params = dict(reoo=1000.0)
+    print(f"[sweep] case={name}", flush=True)
+    c = _case(blocks, grids, states, params)
+    o = oracle_for(blocks, grids, states, params, quirk_send2=0)
+    for d in range(3):
+        got = c.deriv(d, fields)
+        want = o.deriv(d, fields)
+        for g, w in zip(got, want):
+            assert np.max(np.abs(g - w)) <= 1e-13 * np.max(np.abs(w)), (name, "deriv", d)
+        got = c.filter_sweep(d, fields)
+        want = o.filter(d, fields)
+        for g, w in zip(got, want):
+            assert np.max(np.abs(g - w)) <= 1e-13 * np.max(np.abs(w)), (name, "filter", d)
+    c.close()
+    o.close()
+
+
+def test_metrics_match_oracle():
+    # periodic (45) faces between the two blocks carry the coordinate jump
+    blocks, grids = two_block_x(code=45)
+    # shear the grids so every metric term is non-trivial
+    sheared = []
+    for g in grids:
+        x, y, z = g.copy()
+        sheared.append(np.stack([x + 0.05 * np.sin(y), y + 0.03 * np.cos(x), z + 0.02 * np.sin(x + y)]))
+    states = [cases.uniform_state(g.shape[1]) for g in sheared]
+    print("[metrics] two_block sheared", flush=True)
+    c = _case(blocks, sheared, states, dict(reoo=1000.0))
+    o = oracle_for(blocks, sheared, states, dict(reoo=1000.0))
+    for b in range(2):
+        for got, want in zip(c.metrics(b), o.metrics(b)):
+            assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+    c.close()
+    o.close()
+
+
+@pytest.mark.parametrize("n,steps,pert", [(16, 12, 0.0), (24, 12, 1e-3)])
+def test_tgv_steps_match_oracle(n, steps, pert):
+    blocks, grids, states, params = tgv_case(n, pert=pert)
+    print(f"[steps] tgv n={n} steps={steps}", flush=True)
+    c = _case(blocks, grids, states, params)
+    o = oracle_for(blocks, grids, states, params)
+    c.step(steps)
+    o.run(steps)
+    err = rel_err(c.state(0), o.state(0))
+    print("  rel err per variable", err)
+    assert np.all(err < 1e-12), err
+    cg, co = c.clock(), o.clock()
+    assert cg["n"] == co["n"] == steps
+    assert abs(cg["timo"] - co["timo"]) <= 1e-14 * max(co["timo"], 1.0)
+    assert abs(cg["dte"] - co["dte"]) <= 1e-14 * co["dte"]
+    c.close()
+    o.close()
+
+
+def test_noncubic_periodic_matches_oracle():
+    blocks, grids, states, params = tgv_case(0, nxyz=(20, 16, 24), pert=1e-3)
+    print("[steps] noncubic 20x16x24", flush=True)
+    c = _case(blocks, grids, states, params)
+    o = oracle_for(blocks, grids, states, params)
+    c.step(8)
+    o.run(8)
+    err = rel_err(c.state(0), o.state(0))
+    print("  rel err per variable", err)
+    assert np.all(err < 1e-12), err
+    c.close()
+    o.close()
+
+
+def test_multiblock_gcbc_sponge_matches_oracle():
+    """Two sheared blocks joined by a BC 35 interface, non-reflective outer
+    faces, periodic z, sponge, uniform Mach 0.5 flow with a pressure pulse."""
+    sizes = [(24, 16, 14), (16, 16, 14)]
+    blocks = cases.sheared_pair_blocks(sizes)
+    grids = cases.sheared_pair_grids(sizes, h=1.0, amp=0.05)
+    states = [cases.pulse_state(g, mach=0.5, centres=[(20.0, 8.0, 7.0)], amp=1e-3, width=3.0) for g in grids]
+    params = dict(reoo=200.0, amach1=0.5, nsmf=1, cfl=0.8, sponge=1, szco=0.5, szth0=6.0, szth1=6.0,
+                  doml0=0.0, doml1=40.0)
+    print("[steps] multiblock gcbc sponge", flush=True)
+    c = _case(blocks, grids, states, params)
+    o = oracle_for(blocks, grids, states, params)
+    c.step(6)
+    o.run(6)
+    for b in range(2):
+        err = rel_err(c.state(b) - states[b], o.state(b) - states[b])
+        print("  block", b, "rel err of the change", err)
+        assert np.all(err < 1e-11), err
+        err = rel_err(c.state(b), o.state(b))
+        assert np.all(err < 1e-12), err
+    c.close()
+    o.close()
+
+
+def test_ensemble_instances_match_separate_oracles():
+    """Three independent instances of different size, Mach and Re in one
+    context; each must equal its own oracle run."""
+    specs = [(16, 0.1, 1600.0), (20, 0.2, 800.0), (14, 0.05, 3000.0)]
+    blocks, grids, states, params = [], [], [], []
+    for i, (n, m, re) in enumerate(specs):
+        b, g, s, p = tgv_case(n, mach=m, re=re, pert=1e-3, seed=i)
+        b[0]["inst"] = i
+        b[0]["mcd"] = [[i, i]] * 3
+        blocks += b; grids += g; states += s; params.append(p)
+    print("[steps] ensemble of 3", flush=True)
+    c = _case(blocks, grids, states, params)
+    c.step(6)
+    for i, (n, m, re) in enumerate(specs):
+        b, g, s, p = tgv_case(n, mach=m, re=re, pert=1e-3, seed=i)
+        o = oracle_for(b, g, s, p)
+        o.run(6)
+        err = rel_err(c.state(i), o.state(0))
+        print("  instance", i, err)
+        assert np.all(err < 1e-12), err
+        assert c.clock(i)["n"] == 6
+        o.close()
+    c.close()
+
+
+def test_graph_replay_equals_eager():
+    blocks, grids, states, params = tgv_case(16, pert=1e-3)
+    print("[graph] 16^3", flush=True)
+    a = _case(blocks, grids, states, params)
+    b = _case(blocks, grids, states, params)
+    b.use_graph(True)
+    for k in range(3):
+        a.step(4)
+        b.step(4)
+        qa_, qb_ = a.state(0), b.state(0)     # eager work between replays
+        assert np.array_equal(qa_, qb_), k
+    assert a.clock() == b.clock()
+    a.close()
+    b.close()
+
+
+def test_stage_by_stage_api_equals_step():
+    blocks, grids, states, params = tgv_case(16, pert=1e-3)
+    a = _case(blocks, grids, states, params)
+    b = _case(blocks, grids, states, params)
+    a.step(3)
+    for _ in range(3):
+        b.filter()
+        for nk in range(1, 5):
+            b.rk_stage(nk)
+        b.end_step()
+    assert np.array_equal(a.state(0), b.state(0))
+    a.close()
+    b.close()
