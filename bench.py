@@ -7,9 +7,14 @@ Synthetic inputs are generated on the host and uploaded before timing; the
 timed region is K whole time steps (filter phase + 4 RK stages each) with the
 state resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--n 256] [--precision fp64|mixed]
+    python bench.py [--steps K] [--warmup W] [--n 256] [--precision fp64|mixed] [--compare-mixed 0|1]
 
 Prints one JSON line.  value = nominal cells x 4 RK stages x K / seconds.
+With --compare-mixed 1 (the default for an FP64 run) the same workload is
+then run in mixed precision (FP32 line solves with FP64 refinement, FP32
+gradients and stresses) for the same W + K steps; the line's "mixed" object
+gives its rate, its speed-up over FP64 and the max-norm relative error of
+each conserved variable against the FP64 state after W + K steps.
 """
 import argparse
 import json
@@ -49,6 +54,11 @@ def parse():
                     help="replay whole steps as a hipGraph (section timers then stay empty)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same as --n)")
+    ap.add_argument("--compare-mixed", type=int, default=1,
+                    help="FP64 runs: also time mixed precision and report its error against FP64")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run 3 device-to-device copies of MIB MiB (a known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
     return ap.parse_args()
 
 
@@ -97,37 +107,67 @@ def main():
         sys.exit(2)
     from canard_amd.solver import CND_FP64, CND_MIXED, Case
 
+    if a.copy_calibration > 0:
+        import torch
+        nbytes = a.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: 3 copies of {nbytes} bytes", file=sys.stderr, flush=True)
+        del src, dst
+
     n = a.n
     from canard_amd import cases
     blocks = [cases.periodic_block(n)]
     xyz = cases.gridtoy(n, n, n, doml0=0.0, doml1=2 * np.pi, domh=np.pi, span=2 * np.pi)
-    grids = [xyz]
-    states = [cases.tgv_state(xyz, (n + 1, n + 1, n + 1), mach=0.1, pert=1e-3, seed=20261202)]
+    q0 = cases.tgv_state(xyz, (n + 1, n + 1, n + 1), mach=0.1, pert=1e-3, seed=20261202)
     params = dict(fltk=0.49, fltrbc=0.9, reoo=1600.0 / 0.1, tempoo=300.0, cfl=0.8, viscous=1)
     prec = CND_FP64 if a.precision == "fp64" else CND_MIXED
-    print(f"[bench] {n}^3 TGV, {a.precision}: set-up", file=sys.stderr, flush=True)
-    c = Case(blocks, params, precision=prec)
-    c.set_grids(grids)
-    c.set_states(states)
-    c.set_clock()
-    del grids, states
-    c.use_graph(bool(a.graph))
-    c.step(a.warmup)
-    c.synchronize()
-    print(f"[bench] warm-up done, timing {a.steps} steps", file=sys.stderr, flush=True)
-    c.profile(True)
-    t0 = time.perf_counter()
-    c.step(a.steps)
-    c.synchronize()
-    dt = time.perf_counter() - t0
-    sections = c.profile_read()
-    c.profile(False)
-    clock = c.clock()
-    q = c.state(0)
-    finite = bool(np.all(np.isfinite(q)))
-    points, cells = c.points, c.cells
-    c.close()
 
+    def run(precision):
+        """W warm-up steps, then K timed steps; returns (seconds, section
+        timers, clock, final state, stored points, cells)."""
+        c = Case(blocks, params, precision=precision)
+        c.set_grids([xyz])
+        c.set_states([q0])
+        c.set_clock()
+        c.use_graph(bool(a.graph))
+        c.step(a.warmup)
+        c.synchronize()
+        print(f"[bench] warm-up done, timing {a.steps} steps", file=sys.stderr, flush=True)
+        c.profile(True)
+        t0 = time.perf_counter()
+        c.step(a.steps)
+        c.synchronize()
+        dt = time.perf_counter() - t0
+        sections = c.profile_read()
+        c.profile(False)
+        clock = c.clock()
+        q = c.state(0)
+        points, cells = c.points, c.cells
+        c.close()
+        return dt, sections, clock, q, points, cells
+
+    print(f"[bench] {n}^3 TGV, {a.precision}: set-up", file=sys.stderr, flush=True)
+    dt, sections, clock, q, points, cells = run(prec)
+    finite = bool(np.all(np.isfinite(q)))
+    mixed = None
+    if prec == CND_FP64 and a.compare_mixed:
+        print(f"[bench] {n}^3 TGV, mixed: set-up", file=sys.stderr, flush=True)
+        dtm, _, _, qm, _, _ = run(CND_MIXED)
+        err = []
+        for m in range(5):   # q is (5, points)
+            ref = np.max(np.abs(q[1:4])) if 1 <= m <= 3 else np.max(np.abs(q[m]))
+            err.append(float(np.max(np.abs(qm[m] - q[m])) / ref))
+        mixed = {"value": cells * 4 * a.steps / dtm, "ms_per_step": dtm / a.steps * 1e3,
+                 "speedup_vs_fp64": dt / dtm, "steps_compared": a.warmup + a.steps,
+                 "max_rel_err_per_variable": err, "error_target": 1e-6,
+                 "note": "FP32 line solves with one FP64 residual refinement per sweep; gradients and stresses "
+                         "stored in FP32; state, metrics, RK update and GCBC in FP64. Error: max-norm over the "
+                         "field, momentum components against the largest momentum component."}
+        del qm
     value = cells * 4 * a.steps / dt
     # dominant kernel group by measured time
     timed = {k: v for k, v in sections.items() if k in SECTION_BYTES and v[1] > 0}
@@ -163,6 +203,8 @@ def main():
         "final_clock": clock,
         "finite": finite,
     }
+    if mixed is not None:
+        out["mixed"] = mixed
     print(f"[bench] GPU {dt:.2f} s; CPU baseline", file=sys.stderr, flush=True)
     if a.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n or n, a.cpu_steps)

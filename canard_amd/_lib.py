@@ -1,13 +1,15 @@
 """ctypes binding of libcnd.so (include/cnd.h).
 
-The library is loaded from this package directory only; if it is missing the
-import of the solver fails loudly (there is no CPU fallback in the product).
+The library is loaded from this package directory (or from the file that
+CANARD_LIB names, e.g. a build with other tile constants); if it is missing
+the import of the solver fails loudly (there is no CPU fallback in the
+product).
 """
 import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libcnd.so")
+LIBPATH = os.environ.get("CANARD_LIB") or os.path.join(HERE, "libcnd.so")
 
 CND_FP64, CND_MIXED = 0, 1
 # section timers (include/cnd.h CND_SEC_*)

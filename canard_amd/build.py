@@ -23,7 +23,7 @@ ARCH = os.environ.get("CND_ARCH", "gfx950")
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value"]
-SOURCES = ["cnd_scheme.cpp", "cnd_gpu.hip", "cnd_mixed.hip", "cnd_closure.hip"]
+SOURCES = ["cnd_scheme.cpp", "cnd_gpu.hip", "cnd_closure.hip"]
 
 
 def _run(cmd, cwd=None):
@@ -41,23 +41,28 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build_lib(verbose=False):
+def build_lib(verbose=False, lib=LIB, defines=()):
+    """Compile the HIP sources into lib; defines (e.g. "CND_CHUNK_ETA=16")
+    override compile-time tile constants for tuning builds."""
     objs = []
-    os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
+    objdir = os.path.join(HERE, "_obj")
+    if lib != LIB:
+        objdir = os.path.join(objdir, os.path.basename(lib).replace(".", "_"))
+    os.makedirs(objdir, exist_ok=True)
     for s in SOURCES:
         src = os.path.join(CSRC, s)
         if not os.path.exists(src):
             continue
-        obj = os.path.join(HERE, "_obj", s + ".o")
+        obj = os.path.join(objdir, s + ".o")
         lang = ["-x", "hip"] if s.endswith(".hip") else []
-        out = _run([hipcc()] + HIPFLAGS + lang + ["-c", src, "-o", obj])
+        out = _run([hipcc()] + HIPFLAGS + ["-D" + d for d in defines] + lang + ["-c", src, "-o", obj])
         if verbose and out.strip():
             print(out)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def build_fortran(verbose=False):

@@ -13,15 +13,17 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cnd.h"
 #include "cnd_dev.hpp"
 #include "cnd_kernels.hpp"
-#include "cnd_mixed.hpp"
 #include "cnd_scheme.hpp"
+#include "cnd_seg.hpp"
 #include "cnd_sweep.hpp"
 
 using namespace cnd;
@@ -36,10 +38,16 @@ template <int NV>
 struct SrcField {   // NV plain fields, vs apart
     const double *a;
     int64_t vs;
-    __device__ void eval(int64_t g, int, double (&v)[NV]) const
+    struct Raw { double v[NV]; };
+    __device__ void load(int64_t g, int, Raw &r) const
     {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] = a[k * vs + g];
+        for (int k = 0; k < NV; ++k) r.v[k] = a[k * vs + g];
+    }
+    __device__ void finish(const Raw &r, int, double (&v)[NV]) const
+    {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = r.v[k];
     }
 };
 
@@ -48,32 +56,54 @@ struct SrcField {   // NV plain fields, vs apart
 struct SrcPrim {
     const double *q;
     int64_t NT;
-    __device__ void eval(int64_t g, int, double (&o)[4]) const
+    struct Raw { double q[5]; };
+    __device__ void load(int64_t g, int, Raw &r) const
     {
-        const double ri = 1.0 / q[g];
-        const double q2 = q[NT + g], q3 = q[2 * NT + g], q4 = q[3 * NT + g];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) r.q[m] = q[m * NT + g];
+    }
+    __device__ void finish(const Raw &r, int, double (&o)[4]) const
+    {
+        const double ri = 1.0 / r.q[0];
+        const double q2 = r.q[1], q3 = r.q[2], q4 = r.q[3];
         const double u = q2 * ri, v = q3 * ri, w = q4 * ri;
-        const double p = kGm1 * (q[4 * NT + g] - 0.5 * ((q2 * u + q3 * v) + q4 * w));
+        const double p = kGm1 * (r.q[4] - 0.5 * ((q2 * u + q3 * v) + q4 * w));
         o[0] = u; o[1] = v; o[2] = w;
         o[3] = (kGam * p) * ri;
     }
 };
 
-// The five flux components of direction d (src/mo_physics.f90:319-382).
+// The five flux components of direction d (src/mo_physics.f90:319-382);
+// TT is the storage type of the stresses (double, or float in mixed mode).
+template <class TT>
 struct SrcFlux {
-    const double *q, *tau, *met;
+    const double *q;
+    const TT *tau;
+    const double *met;
     const BlkDev *blks;
     const InstDev *inst;
     int64_t NT;
     int d, vis;
-    __device__ void eval(int64_t g, int b, double (&o)[5]) const
+    struct Raw { double q[5], m[3], t[9]; };
+    __device__ void load(int64_t g, int, Raw &r) const
+    {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) r.q[k] = q[k * NT + g];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r.m[k] = met[(3 * d + k) * NT + g];
+        if (vis) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) r.t[k] = double(tau[k * NT + g]);
+        }
+    }
+    __device__ void finish(const Raw &r, int b, double (&o)[5]) const
     {
         const double *umf = inst[blks[b].inst].umf;
-        const double q1 = q[g], q2 = q[NT + g], q3 = q[2 * NT + g], q4 = q[3 * NT + g], q5 = q[4 * NT + g];
+        const double q1 = r.q[0], q2 = r.q[1], q3 = r.q[2], q4 = r.q[3], q5 = r.q[4];
         const double ri = 1.0 / q1;
         const double u = q2 * ri, v = q3 * ri, w = q4 * ri;
         const double p = kGm1 * (q5 - 0.5 * ((q2 * u + q3 * v) + q4 * w));
-        const double m0 = met[(3 * d + 0) * NT + g], m1 = met[(3 * d + 1) * NT + g], m2 = met[(3 * d + 2) * NT + g];
+        const double m0 = r.m[0], m1 = r.m[1], m2 = r.m[2];
         const double s = (m0 * (u + umf[0]) + m1 * (v + umf[1])) + m2 * (w + umf[2]);
         o[0] = q1 * s;
         o[1] = q2 * s + m0 * p;
@@ -82,27 +112,26 @@ struct SrcFlux {
         o[4] = (q5 + p) * s - p * ((umf[0] * m0 + umf[1] * m1) + umf[2] * m2);
         if (vis) {
             // stress rows x (txx, txy, tzx), y (txy, tyy, tyz), z (tzx, tyz, tzz)
-            const double txx = tau[g], tyy = tau[NT + g], tzz = tau[2 * NT + g];
-            const double txy = tau[3 * NT + g], tyz = tau[4 * NT + g], tzx = tau[5 * NT + g];
+            const double txx = r.t[0], tyy = r.t[1], tzz = r.t[2], txy = r.t[3], tyz = r.t[4], tzx = r.t[5];
             o[1] = o[1] - m0 * txx; o[1] = o[1] - m1 * txy; o[1] = o[1] - m2 * tzx;
             o[2] = o[2] - m0 * txy; o[2] = o[2] - m1 * tyy; o[2] = o[2] - m2 * tyz;
             o[3] = o[3] - m0 * tzx; o[3] = o[3] - m1 * tyz; o[3] = o[3] - m2 * tzz;
-            o[4] = o[4] - m0 * tau[6 * NT + g]; o[4] = o[4] - m1 * tau[7 * NT + g];
-            o[4] = o[4] - m2 * tau[8 * NT + g];
+            o[4] = o[4] - m0 * r.t[6]; o[4] = o[4] - m1 * r.t[7]; o[4] = o[4] - m2 * r.t[8];
         }
     }
 };
 
 // ---- sinks ----------------------------------------------------------------
-template <int NV>
-struct SinkOut {   // out fields vs apart
-    double *o;
+template <int NV, class TO = double>
+struct SinkOut {   // out fields vs apart, stored as TO
+    TO *o;
     int64_t vs;
     __device__ void store(int64_t g, int, const double (&x)[NV])
     {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) o[k * vs + g] = x[k];
+        for (int k = 0; k < NV; ++k) o[k * vs + g] = TO(x[k]);
     }
+    __device__ void put(int64_t g, int, int k, double x) { o[k * vs + g] = TO(x); }
     __device__ void ends(int, int, int, double) {}
 };
 
@@ -116,6 +145,7 @@ struct SinkFilt {  // out = f + correction
 #pragma unroll
         for (int k = 0; k < NV; ++k) o[k * vs + g] = f[k * vs + g] + x[k];
     }
+    __device__ void put(int64_t g, int, int k, double x) { o[k * vs + g] = f[k * vs + g] + x; }
     __device__ void ends(int, int, int, double) {}
 };
 
@@ -128,68 +158,143 @@ struct SinkFlux {  // de = (de +) dF/dxi_d; pencil ends to drva
 #pragma unroll
         for (int m = 0; m < 5; ++m) de[m * NT + g] = first ? x[m] : de[m * NT + g] + x[m];
     }
+    __device__ void put(int64_t g, int, int m, double x) { de[m * NT + g] = first ? x : de[m * NT + g] + x; }
     __device__ void ends(int side, int lane, int m, double x) { drva[(side * 5 + m) * FT + fi0 + lane] = x; }
 };
 
 // ---- sweep kernels -----------------------------------------------------------
-// grid: (tiles of direction d, variable groups); NV waves per tile of 64
-// pencils; dynamic LDS = sweep_lds_bytes<NV>().
-template <int NV>
-constexpr size_t lds_bytes() { return sweep_lds_bytes<NV>(); }
+// grid: (xcd_width(tiles of direction d), variable groups); NV waves per tile
+// of 64 pencils; C = chunk_of(d); dynamic LDS = sweep_lds_bytes<NV, C>().
+#define SWEEP_TILE_PROLOGUE(ntiles)        \
+    const int tix = xcd_tile(ntiles);      \
+    if (tix >= (ntiles)) return;           \
+    extern __shared__ double lds[];
 
+template <int C>
 __global__ __launch_bounds__(kTile) void k_fd_field(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
                                                    const double *__restrict__ lu, const Tile *__restrict__ tiles,
                                                    int ntiles, int d, int n45, int nmax, double *ys,
                                                    const double *in, int64_t in_vstride, double *out,
                                                    int64_t out_vstride)
 {
-    extern __shared__ double lds[];
+    SWEEP_TILE_PROLOGUE(ntiles)
     const int v = blockIdx.y;
     SrcField<1> src{in + v * in_vstride, 0};
     SinkOut<1> sink{out + v * out_vstride, 0};
-    sweep_tile<1, 0>(blks, tabs, lu, tiles[blockIdx.x], int64_t(v) * ntiles + blockIdx.x, d, n45, nmax, ys, src,
-                     sink, lds);
+    sweep_tile<1, 0, C>(blks, tabs, lu, tiles[tix], int64_t(v) * ntiles + tix, d, n45, nmax, ys, src, sink, lds);
 }
 
+template <int C>
 __global__ __launch_bounds__(kTile) void k_filter_field(const BlkDev *__restrict__ blks,
                                                        const TabDev *__restrict__ tabs,
                                                        const double *__restrict__ lu,
                                                        const Tile *__restrict__ tiles, int ntiles, int d, int nmax,
                                                        double *ys, const double *in, double *out, int64_t vstride)
 {
-    extern __shared__ double lds[];
+    SWEEP_TILE_PROLOGUE(ntiles)
     const int v = blockIdx.y;
     SrcField<1> src{in + v * vstride, 0};
     SinkFilt<1> sink{out + v * vstride, in + v * vstride, 0};
-    sweep_tile<1, 1>(blks, tabs, lu, tiles[blockIdx.x], int64_t(v) * ntiles + blockIdx.x, d, 0, nmax, ys, src,
-                     sink, lds);
+    sweep_tile<1, 1, C>(blks, tabs, lu, tiles[tix], int64_t(v) * ntiles + tix, d, 0, nmax, ys, src, sink, lds);
 }
 
 // Velocity and temperature gradients along d (the 12 derivative sweeps of
 // calc_viscous_shear_stress, src/mo_physics.f90:238-280).
+template <int C>
 __global__ __launch_bounds__(4 * kTile) void k_fd_grad(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
                                                   const double *__restrict__ lu, const Tile *__restrict__ tiles,
-                                                  int d, int nmax, double *ys, const double *q, double *grad,
-                                                  int64_t NT)
+                                                  int ntiles, int d, int nmax, double *ys, const double *q,
+                                                  double *grad, int64_t NT)
 {
-    extern __shared__ double lds[];
+    SWEEP_TILE_PROLOGUE(ntiles)
     SrcPrim src{q, NT};
     SinkOut<4> sink{grad + 4 * d * NT, NT};
-    sweep_tile<4, 0>(blks, tabs, lu, tiles[blockIdx.x], blockIdx.x, d, 0, nmax, ys, src, sink, lds);
+    sweep_tile<4, 0, C>(blks, tabs, lu, tiles[tix], tix, d, 0, nmax, ys, src, sink, lds);
 }
 
 // Flux derivative along d accumulated into de (src/mo_physics.f90:385-399).
+template <int C>
 __global__ __launch_bounds__(5 * kTile) void k_fd_flux(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
                                                   const InstDev *__restrict__ inst, const double *__restrict__ lu,
-                                                  const Tile *__restrict__ tiles, int d, int nmax, double *ys,
-                                                  const double *q, const double *tau, const double *met, double *de,
-                                                  double *drva, int64_t NT, int64_t FT, int vis)
+                                                  const Tile *__restrict__ tiles, int ntiles, int d, int nmax,
+                                                  double *ys, const double *q, const double *tau, const double *met,
+                                                  double *de, double *drva, int64_t NT, int64_t FT, int vis)
 {
-    extern __shared__ double lds[];
-    const Tile t = tiles[blockIdx.x];
-    SrcFlux src{q, tau, met, blks, inst, NT, d, vis};
+    SWEEP_TILE_PROLOGUE(ntiles)
+    const Tile t = tiles[tix];
+    SrcFlux<double> src{q, tau, met, blks, inst, NT, d, vis};
     SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0, d == 0};
-    sweep_tile<5, 0>(blks, tabs, lu, t, blockIdx.x, d, 0, nmax, ys, src, sink, lds);
+    sweep_tile<5, 0, C>(blks, tabs, lu, t, tix, d, 0, nmax, ys, src, sink, lds);
+}
+
+// ---- segmented sweeps (cnd_seg.hpp) ---------------------------------------
+// xi: tiles of 4 pencils, segments of <= 17 points, 16-point staging runs
+// along the pencil; eta/zeta: tiles of 16 pencils, segments of <= 33 points,
+// staging runs across 16 neighbouring pencils.  SPMAX bounds the segments per
+// pencil (and so the longest pencil these shapes take).
+struct SegXi { static constexpr int L = 17, P = 4, C = 16, SPMAX = 16; static constexpr bool XI = true; };
+#ifndef CND_SEG_ETA_C
+#define CND_SEG_ETA_C 8
+#endif
+struct SegEta { static constexpr int L = 33, P = 16, C = CND_SEG_ETA_C, SPMAX = 8; static constexpr bool XI = false; };
+
+#define SEG_PROLOGUE(ntiles)               \
+    const int tix = xcd_tile(ntiles);      \
+    if (tix >= (ntiles)) return;           \
+    extern __shared__ double lds[];
+
+template <class G, bool MIX>
+__global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_fd_field(
+    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
+    int ntiles, int d, int n45, int SP, const double *in, int64_t in_vstride, double *out, int64_t out_vstride)
+{
+    SEG_PROLOGUE(ntiles)
+    const int v = blockIdx.y;
+    SrcField<1> src{in + v * in_vstride, 0};
+    SinkOut<1> sink{out + v * out_vstride, 0};
+    seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX>(blks, tabs, tb, tiles[tix], d, n45, SP, src, sink, lds);
+}
+
+template <class G, bool MIX>
+__global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_filter_field(
+    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
+    int ntiles, int d, int SP, const double *in, double *out, int64_t vstride)
+{
+    SEG_PROLOGUE(ntiles)
+    const int v = blockIdx.y;
+    SrcField<1> src{in + v * vstride, 0};
+    SinkFilt<1> sink{out + v * vstride, in + v * vstride, 0};
+    seg_tile<1, 1, G::L, G::P, G::C, G::XI, MIX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
+}
+
+// Velocity and temperature gradients along d; in mixed mode they are stored
+// in FP32.
+template <class G, bool MIX>
+__global__ __launch_bounds__(4 * G::P * G::SPMAX) void k_seg_grad(
+    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
+    int ntiles, int d, int SP, const double *q, typename std::conditional<MIX, float, double>::type *grad,
+    int64_t NT)
+{
+    SEG_PROLOGUE(ntiles)
+    using TG = typename std::conditional<MIX, float, double>::type;
+    SrcPrim src{q, NT};
+    SinkOut<4, TG> sink{grad + 4 * d * NT, NT};
+    seg_tile<4, 0, G::L, G::P, G::C, G::XI, MIX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
+}
+
+template <class G, bool MIX>
+__global__ __launch_bounds__(5 * G::P * G::SPMAX) void k_seg_flux(
+    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, const InstDev *__restrict__ inst, SegTabs tb,
+    const Tile *__restrict__ tiles, int ntiles, int d, int SP, const double *q,
+    const typename std::conditional<MIX, float, double>::type *tau, const double *met, double *de, double *drva,
+    int64_t NT, int64_t FT, int vis)
+{
+    SEG_PROLOGUE(ntiles)
+    using TT = typename std::conditional<MIX, float, double>::type;
+    const Tile t = tiles[tix];
+    SrcFlux<TT> src{q, tau, met, blks, inst, NT, d, vis};
+    SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0, d == 0};
+    seg_tile<5, 0, G::L, G::P, G::C, G::XI, MIX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
 }
 
 // ---- pointwise kernels ------------------------------------------------------
@@ -202,10 +307,12 @@ __global__ __launch_bounds__(5 * kTile) void k_fd_flux(const BlkDev *__restrict_
     const int64_t g = B.off + lp;
 
 // Viscous stresses and heat flux (src/mo_physics.f90:245-299).
+// TS: storage type of the gradients and stresses (float in mixed mode).
+template <class TS>
 __global__ __launch_bounds__(kPts) void k_stress(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
                                                 const Tile *__restrict__ tiles, const double *__restrict__ q,
-                                                const double *__restrict__ grad, const double *__restrict__ met,
-                                                double *__restrict__ tau, int64_t NT)
+                                                const TS *__restrict__ grad, const double *__restrict__ met,
+                                                TS *__restrict__ tau, int64_t NT)
 {
     POINT_PROLOGUE
     const InstDev &I = inst[B.inst];
@@ -215,7 +322,7 @@ __global__ __launch_bounds__(kPts) void k_stress(const BlkDev *__restrict__ blks
     const double p = kGm1 * (q[4 * NT + g] - 0.5 * ((q2 * u + q3 * v) + q4 * w));
     const double T = (kGam * p) * ri;
     const double mu = (I.srefp1dre * pow(T, 1.5)) / (T + I.srefoo);
-    auto G = [&](int d, int var) { return grad[(4 * d + var) * NT + g]; };
+    auto G = [&](int d, int var) { return double(grad[(4 * d + var) * NT + g]); };
     auto CT = [&](int c, int var) {
         return (met[c * NT + g] * G(0, var) + met[(3 + c) * NT + g] * G(1, var)) + met[(6 + c) * NT + g] * G(2, var);
     };
@@ -228,11 +335,11 @@ __global__ __launch_bounds__(kPts) void k_stress(const BlkDev *__restrict__ blks
     const double d5 = kTwoThirds * ((vtxx + vtyy) + vtzz);
     const double txx = r1 * ((vtxx + vtxx) - d5), tyy = r1 * ((vtyy + vtyy) - d5), tzz = r1 * ((vtzz + vtzz) - d5);
     const double txy = r1 * (vtxy + vhzz), tyz = r1 * (vtyz + vhxx), tzx = r1 * (vtzx + vhyy);
-    tau[g] = txx; tau[NT + g] = tyy; tau[2 * NT + g] = tzz;
-    tau[3 * NT + g] = txy; tau[4 * NT + g] = tyz; tau[5 * NT + g] = tzx;
-    tau[6 * NT + g] = ((r2 * s1 + u * txx) + v * txy) + w * tzx;
-    tau[7 * NT + g] = ((r2 * s2 + u * txy) + v * tyy) + w * tyz;
-    tau[8 * NT + g] = ((r2 * s3 + u * tzx) + v * tyz) + w * tzz;
+    tau[g] = TS(txx); tau[NT + g] = TS(tyy); tau[2 * NT + g] = TS(tzz);
+    tau[3 * NT + g] = TS(txy); tau[4 * NT + g] = TS(tyz); tau[5 * NT + g] = TS(tzx);
+    tau[6 * NT + g] = TS(((r2 * s1 + u * txx) + v * txy) + w * tzx);
+    tau[7 * NT + g] = TS(((r2 * s2 + u * txy) + v * tyy) + w * tyz);
+    tau[8 * NT + g] = TS(((r2 * s3 + u * tzx) + v * tyz) + w * tzz);
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -780,6 +887,12 @@ struct cnd_ctx {
     double *dlu = nullptr;
     std::vector<Tile> htiles[3];
     Tile *dtiles[3] = {nullptr, nullptr, nullptr};
+    // segmented sweeps: tiles per direction, segments per pencil (rounded to
+    // whole waves; 0: the direction uses the tiled sweeps), homogeneous tables
+    std::vector<Tile> hstiles[3];
+    Tile *dstiles[3] = {nullptr, nullptr, nullptr};
+    int seg_sp[3] = {0, 0, 0};
+    double *dsegtab = nullptr;
     std::vector<Tile> hpt;
     Tile *dpt = nullptr;
     double *qa = nullptr, *qo = nullptr, *de = nullptr, *grad = nullptr, *tau = nullptr, *met = nullptr;
@@ -787,7 +900,10 @@ struct cnd_ctx {
     int nmax[3] = {0, 0, 0};   // longest pencil per direction
     int64_t ys_len = 0;        // doubles in the forward-value scratch
     double *drva[3] = {}, *cmf[3] = {}, *surf[3] = {};
-    float *fscr = nullptr;   // mixed-precision scratch
+    // mixed precision: FP32 gradients and stresses; FP32 LU / homogeneous
+    // tables and the FP64 banded LHS for the refinement
+    float *gradf = nullptr, *tauf = nullptr, *dlu32 = nullptr, *dseg32 = nullptr;
+    double *dlhs = nullptr;
     JuncDev jc, je;
     std::vector<char> grid_set;
     bool metrics_ready = false, state_in_qo = false;
@@ -848,6 +964,18 @@ int upload_inst(cnd_ctx *c)
 
 // Launch helpers ----------------------------------------------------------------
 inline dim3 tiles_grid(cnd_ctx *c, int d, int y) { return dim3(unsigned(c->htiles[d].size()), unsigned(y)); }
+inline dim3 sweep_grid(cnd_ctx *c, int d, int y) { return dim3(unsigned(xcd_width(int(c->htiles[d].size()))), unsigned(y)); }
+
+// Launch a sweep kernel template K<C> with the chunk length of direction d.
+#define LAUNCH_SWEEP(K, NV, c, d, y, block, ...)                                                           \
+    do {                                                                                                   \
+        if ((d) == 0)                                                                                      \
+            hipLaunchKernelGGL(K<kChunkXi>, sweep_grid(c, d, y), dim3(block),                              \
+                               (sweep_lds_bytes<NV, kChunkXi>()), (c)->st, __VA_ARGS__);                    \
+        else                                                                                               \
+            hipLaunchKernelGGL(K<kChunkEta>, sweep_grid(c, d, y), dim3(block),                             \
+                               (sweep_lds_bytes<NV, kChunkEta>()), (c)->st, __VA_ARGS__);                   \
+    } while (0)
 inline dim3 pts_grid(cnd_ctx *c) { return dim3(unsigned(c->hpt.size())); }
 
 int check_launch(cnd_ctx *c)
@@ -885,18 +1013,52 @@ struct Section {
 };
 
 inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
+inline int nstiles(cnd_ctx *c, int d) { return int(c->hstiles[d].size()); }
 
-int run_filter_field(cnd_ctx *c, int d, const double *in, double *out, int nvar)
+// Launch a segmented sweep K<G, MIX> (NV variables) along d with the shape of d.
+#define LAUNCH_SEG(K, MIX, NV, c, d, y, ...)                                                                \
+    do {                                                                                                   \
+        const int sp_ = (c)->seg_sp[d];                                                                    \
+        const dim3 g_(unsigned(xcd_width(nstiles(c, d))), unsigned(y));                                   \
+        if ((d) == 0)                                                                                      \
+            hipLaunchKernelGGL((K<SegXi, MIX>), g_, dim3((NV) * SegXi::P * sp_),                           \
+                               (SegShape<NV, 0, SegXi::L, SegXi::P, SegXi::C, true>::lds_bytes(sp_)),      \
+                               (c)->st, __VA_ARGS__);                                                      \
+        else                                                                                               \
+            hipLaunchKernelGGL((K<SegEta, MIX>), g_, dim3((NV) * SegEta::P * sp_),                         \
+                               (SegShape<NV, 0, SegEta::L, SegEta::P, SegEta::C, false>::lds_bytes(sp_)),  \
+                               (c)->st, __VA_ARGS__);                                                      \
+    } while (0)
+
+inline SegTabs seg_tabs(cnd_ctx *c) { return SegTabs{c->dlu, c->dsegtab, c->dlhs, c->dlu32, c->dseg32}; }
+
+// mix: FP32 solve with FP64 refinement (segmented sweeps only)
+int run_filter_field(cnd_ctx *c, int d, const double *in, double *out, int nvar, bool mix = false)
 {
-    hipLaunchKernelGGL(k_filter_field, tiles_grid(c, d, nvar), dim3(kTile), lds_bytes<1>(), c->st, c->dblk, c->dtab,
-                       c->dlu, c->dtiles[d], ntiles(c, d), d, c->nmax[d], c->ys, in, out, c->NT);
+    if (mix)
+        LAUNCH_SEG(k_seg_filter_field, true, 1, c, d, nvar, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
+                   nstiles(c, d), d, c->seg_sp[d], in, out, c->NT);
+    else if (c->seg_sp[d] > 0)
+        LAUNCH_SEG(k_seg_filter_field, false, 1, c, d, nvar, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
+                   nstiles(c, d), d, c->seg_sp[d], in, out, c->NT);
+    else
+        LAUNCH_SWEEP(k_filter_field, 1, c, d, nvar, kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d,
+                     c->nmax[d], c->ys, in, out, c->NT);
     return check_launch(c);
 }
 
-int run_fd_field(cnd_ctx *c, int d, int n45, const double *in, int64_t ivs, double *out, int64_t ovs, int nvar)
+int run_fd_field(cnd_ctx *c, int d, int n45, const double *in, int64_t ivs, double *out, int64_t ovs, int nvar,
+                 bool mix = false)
 {
-    hipLaunchKernelGGL(k_fd_field, tiles_grid(c, d, nvar), dim3(kTile), lds_bytes<1>(), c->st, c->dblk, c->dtab,
-                       c->dlu, c->dtiles[d], ntiles(c, d), d, n45, c->nmax[d], c->ys, in, ivs, out, ovs);
+    if (mix)
+        LAUNCH_SEG(k_seg_fd_field, true, 1, c, d, nvar, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d], nstiles(c, d),
+                   d, n45, c->seg_sp[d], in, ivs, out, ovs);
+    else if (c->seg_sp[d] > 0)
+        LAUNCH_SEG(k_seg_fd_field, false, 1, c, d, nvar, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
+                   nstiles(c, d), d, n45, c->seg_sp[d], in, ivs, out, ovs);
+    else
+        LAUNCH_SWEEP(k_fd_field, 1, c, d, nvar, kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d, n45,
+                     c->nmax[d], c->ys, in, ivs, out, ovs);
     return check_launch(c);
 }
 
@@ -907,15 +1069,10 @@ int enqueue_filter(cnd_ctx *c)
 {
     Section sec(c, CND_SEC_FILTER, 3);
     int r;
-    if (c->precision == CND_MIXED) {
-        if ((r = mixed_filter(c->st, c->dblk, c->dtab, c->dlu, c->dtiles, c->htiles, c->qa, c->qo, c->fscr, c->NT, 1))) return fail(c, r, "mixed filter");
-        if ((r = mixed_filter(c->st, c->dblk, c->dtab, c->dlu, c->dtiles, c->htiles, c->qo, c->qa, c->fscr, c->NT, 2))) return fail(c, r, "mixed filter");
-        if ((r = mixed_filter(c->st, c->dblk, c->dtab, c->dlu, c->dtiles, c->htiles, c->qa, c->qo, c->fscr, c->NT, 0))) return fail(c, r, "mixed filter");
-        return CND_OK;
-    }
-    if ((r = run_filter_field(c, 1, c->qa, c->qo, 5))) return r;
-    if ((r = run_filter_field(c, 2, c->qo, c->qa, 5))) return r;
-    if ((r = run_filter_field(c, 0, c->qa, c->qo, 5))) return r;
+    const bool mix = c->precision == CND_MIXED;
+    if ((r = run_filter_field(c, 1, c->qa, c->qo, 5, mix))) return r;
+    if ((r = run_filter_field(c, 2, c->qo, c->qa, 5, mix))) return r;
+    if ((r = run_filter_field(c, 0, c->qa, c->qo, 5, mix))) return r;
     return CND_OK;
 }
 
@@ -937,31 +1094,41 @@ int enqueue_stage(cnd_ctx *c, int nk)
         Section sec(c, CND_SEC_GRAD, 3);
         for (int d = 0; d < 3; ++d) {
             if (c->precision == CND_MIXED) {
-                if ((r = mixed_grad(c->st, c->dblk, c->dtab, c->dlu, c->dtiles[d], unsigned(c->htiles[d].size()), d,
-                                    cur, c->grad, c->fscr, c->NT)))
-                    return fail(c, r, "mixed grad");
+                LAUNCH_SEG(k_seg_grad, true, 4, c, d, 1, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
+                           nstiles(c, d), d, c->seg_sp[d], cur, c->gradf, c->NT);
+            } else if (c->seg_sp[d] > 0) {
+                LAUNCH_SEG(k_seg_grad, false, 4, c, d, 1, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
+                           nstiles(c, d), d, c->seg_sp[d], cur, c->grad, c->NT);
             } else {
-                hipLaunchKernelGGL(k_fd_grad, tiles_grid(c, d, 1), dim3(4 * kTile), lds_bytes<4>(), c->st, c->dblk,
-                                   c->dtab, c->dlu, c->dtiles[d], d, c->nmax[d], c->ys, cur, c->grad, c->NT);
+                LAUNCH_SWEEP(k_fd_grad, 4, c, d, 1, 4 * kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d),
+                             d, c->nmax[d], c->ys, cur, c->grad, c->NT);
             }
         }
     }
     if (c->any_visc) {
         Section sec(c, CND_SEC_STRESS, 1);
-        hipLaunchKernelGGL(k_stress, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->grad,
-                           c->met, c->tau, c->NT);
+        if (c->precision == CND_MIXED)
+            hipLaunchKernelGGL(k_stress<float>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
+                               c->gradf, c->met, c->tauf, c->NT);
+        else
+            hipLaunchKernelGGL(k_stress<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
+                               c->grad, c->met, c->tau, c->NT);
     }
     {
     Section sec(c, CND_SEC_FLUX, 3);
     for (int d = 0; d < 3; ++d) {
         if (c->precision == CND_MIXED) {
-            if ((r = mixed_flux(c->st, c->dblk, c->dtab, c->dinst, c->dlu, c->dtiles[d], unsigned(c->htiles[d].size()), d,
-                                cur, c->tau, c->met, c->de, c->ys, c->drva[d], c->fscr, c->NT, c->FT[d], c->any_visc)))
-                return fail(c, r, "mixed flux");
+            LAUNCH_SEG(k_seg_flux, true, 5, c, d, 1, c->dblk, c->dtab, c->dinst, seg_tabs(c), c->dstiles[d],
+                       nstiles(c, d), d, c->seg_sp[d], cur, c->tauf, c->met, c->de, c->drva[d], c->NT, c->FT[d],
+                       int(c->any_visc));
+        } else if (c->seg_sp[d] > 0) {
+            LAUNCH_SEG(k_seg_flux, false, 5, c, d, 1, c->dblk, c->dtab, c->dinst, seg_tabs(c), c->dstiles[d],
+                       nstiles(c, d), d, c->seg_sp[d], cur, c->tau, c->met, c->de, c->drva[d], c->NT, c->FT[d],
+                       int(c->any_visc));
         } else {
-            hipLaunchKernelGGL(k_fd_flux, tiles_grid(c, d, 1), dim3(5 * kTile), lds_bytes<5>(), c->st, c->dblk,
-                               c->dtab, c->dinst, c->dlu, c->dtiles[d], d, c->nmax[d], c->ys, cur, c->tau, c->met,
-                               c->de, c->drva[d], c->NT, c->FT[d], int(c->any_visc));
+            LAUNCH_SWEEP(k_fd_flux, 5, c, d, 1, 5 * kTile, c->dblk, c->dtab, c->dinst, c->dlu, c->dtiles[d],
+                         ntiles(c, d), d, c->nmax[d], c->ys, cur, c->tau, c->met, c->de, c->drva[d], c->NT, c->FT[d],
+                         int(c->any_visc));
         }
     }
     }
@@ -1067,7 +1234,7 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     for (int i = 0; i < ninst; ++i) build_tables(params[i].fltk, params[i].fltrbc, c->htab[i]);
 
     // layout
-    std::vector<double> lu;
+    std::vector<double> lu, segtab, lhs;
     c->hblk.resize(nblocks);
     int64_t off = 0, foff[3] = {0, 0, 0};
     for (int b = 0; b < nblocks; ++b) {
@@ -1099,11 +1266,20 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
             const int np = npencils(B, d);
             for (int q0 = 0; q0 < np; q0 += kLine) c->htiles[d].push_back(Tile{b, q0});
             foff[d] += np;
+            const int PL = d == 0 ? SegXi::P : SegEta::P, SL = d == 0 ? SegXi::L : SegEta::L;
+            for (int q0 = 0; q0 < np; q0 += PL) c->hstiles[d].push_back(Tile{b, q0});
             for (int sch = 0; sch < 2; ++sch) {
                 B.lu[sch][d] = int(lu.size() / 5);
                 const size_t at = lu.size();
                 lu.resize(at + 5 * size_t(s.n[d]));
                 lu_factors(c->htab[s.inst], sch, s.n[d], B.qd[d][0] != 0, B.qd[d][1] != 0, lu.data() + at);
+                lhs.resize(lu.size());
+                lhs_rows(c->htab[s.inst], sch, s.n[d], B.qd[d][0] != 0, B.qd[d][1] != 0, lhs.data() + at);
+                segtab.resize(4 * (at / 5 + size_t(s.n[d])), 0.0);
+                const int nd = s.n[d], S = seg_count(nd, SL);
+                if (S > 0)
+                    seg_tables(lu.data() + at, S, [&](int k, int &a0, int &l0) { seg_bounds(nd, SL, S, k, a0, l0); },
+                               segtab.data() + 4 * (at / 5));
             }
         }
         for (int64_t p0 = 0; p0 < npt; p0 += kPts) c->hpt.push_back(Tile{b, int(p0)});
@@ -1113,6 +1289,29 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     for (int d = 0; d < 3; ++d) c->FT[d] = foff[d];
     for (int b = 0; b < nblocks; ++b)
         for (int d = 0; d < 3; ++d) c->nmax[d] = std::max(c->nmax[d], c->spec[b].n[d]);
+    // segmented sweeps where every block's pencils of direction d fit the
+    // shape (CANARD_SWEEP=tiled keeps the tiled sweeps everywhere)
+    {
+        const char *sw = std::getenv("CANARD_SWEEP");
+        const bool tiled_only = sw && std::string(sw) == "tiled";
+        for (int d = 0; d < 3; ++d) {
+            const int SL = d == 0 ? SegXi::L : SegEta::L, PL = d == 0 ? SegXi::P : SegEta::P;
+            const int SPM = d == 0 ? SegXi::SPMAX : SegEta::SPMAX, per_wave = 64 / PL;
+            int sp = tiled_only ? 0 : 1;
+            for (int b = 0; b < nblocks && sp > 0; ++b) {
+                const int S = seg_count(c->spec[b].n[d], SL);
+                sp = S == 0 ? 0 : std::max(sp, S);
+            }
+            if (sp > 0) sp = (sp + per_wave - 1) / per_wave * per_wave;
+            c->seg_sp[d] = sp <= SPM ? sp : 0;
+            if (precision == CND_MIXED && c->seg_sp[d] == 0) {
+                delete c;
+                return fail(nullptr, CND_EINVAL,
+                            "mixed precision needs the segmented sweeps in every direction (pencils of 13..272 (xi) "
+                            "or 13..264 (eta, zeta) points, and CANARD_SWEEP not set to tiled)");
+            }
+        }
+    }
     // forward values of up to 9 variables per pencil (metric filtering)
     for (int d = 0; d < 3; ++d)
         c->ys_len = std::max(c->ys_len, int64_t(9) * int64_t(c->htiles[d].size()) * kTile * c->nmax[d]);
@@ -1131,11 +1330,14 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
                 ok(dalloc(c, &c->dlu, lu.size())) && ok(dalloc(c, &c->qa, 5 * NT)) && ok(dalloc(c, &c->qo, 5 * NT)) &&
                 ok(dalloc(c, &c->de, 5 * NT)) && ok(dalloc(c, &c->grad, 12 * NT)) && ok(dalloc(c, &c->tau, 9 * NT)) &&
                 ok(dalloc(c, &c->met, 10 * NT)) && ok(dalloc(c, &c->ys, c->ys_len)) && ok(dalloc(c, &c->xyz, 3 * NT)) &&
-                ok(dalloc(c, &c->sp_par, 5 * ninst));
+                ok(dalloc(c, &c->sp_par, 5 * ninst)) && ok(dalloc(c, &c->dsegtab, segtab.size()));
     if (good && c->any_sponge) good = ok(dalloc(c, &c->sponge, 2 * NT));
-    if (good && precision == CND_MIXED) good = ok(dalloc(c, &c->fscr, mixed_scratch_floats(NT)));
+    if (good && precision == CND_MIXED)
+        good = ok(dalloc(c, &c->gradf, 12 * NT)) && ok(dalloc(c, &c->tauf, 9 * NT)) &&
+               ok(dalloc(c, &c->dlu32, lu.size())) && ok(dalloc(c, &c->dseg32, segtab.size())) &&
+               ok(dalloc(c, &c->dlhs, lhs.size()));
     for (int d = 0; good && d < 3; ++d)
-        good = ok(dalloc(c, &c->dtiles[d], c->htiles[d].size())) && ok(dalloc(c, &c->drva[d], 10 * c->FT[d])) &&
+        good = ok(dalloc(c, &c->dtiles[d], c->htiles[d].size())) && ok(dalloc(c, &c->dstiles[d], c->hstiles[d].size())) && ok(dalloc(c, &c->drva[d], 10 * c->FT[d])) &&
                ok(dalloc(c, &c->cmf[d], 6 * c->FT[d])) && ok(dalloc(c, &c->surf[d], 10 * c->FT[d]));
     if (good) good = ok(dalloc(c, &c->dpt, c->hpt.size())) && ok(dalloc(c, &c->tmax, 2 * c->hpt.size()));
     if (!good) { std::string m = c->err; cnd_destroy(c); return fail(nullptr, r ? r : CND_ENOMEM, m); }
@@ -1166,6 +1368,14 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     up(c->dlu, lu.data(), sizeof(double) * lu.size());
     up(c->sp_par, sp.data(), sizeof(double) * sp.size());
     for (int d = 0; d < 3; ++d) up(c->dtiles[d], c->htiles[d].data(), sizeof(Tile) * c->htiles[d].size());
+    for (int d = 0; d < 3; ++d) up(c->dstiles[d], c->hstiles[d].data(), sizeof(Tile) * c->hstiles[d].size());
+    up(c->dsegtab, segtab.data(), sizeof(double) * segtab.size());
+    if (precision == CND_MIXED) {
+        const std::vector<float> lu32(lu.begin(), lu.end()), seg32(segtab.begin(), segtab.end());
+        up(c->dlu32, lu32.data(), sizeof(float) * lu32.size());
+        up(c->dseg32, seg32.data(), sizeof(float) * seg32.size());
+        up(c->dlhs, lhs.data(), sizeof(double) * lhs.size());
+    }
     up(c->dpt, c->hpt.data(), sizeof(Tile) * c->hpt.size());
     if (!r) r = upload_inst(c);
     if (!r) {
@@ -1206,11 +1416,12 @@ void cnd_destroy(cnd_ctx *c)
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->tau, c->met, c->ys,
-                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->fscr, c->jc.tgt, c->jc.src, c->jc.first, c->jc.scale,
-                    c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp};
+                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->tauf, c->dlu32, c->dseg32, c->dlhs, c->jc.tgt, c->jc.src, c->jc.first, c->jc.scale,
+                    c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp, c->dsegtab};
     for (void *p : ptrs) if (p) hipFree(p);
     for (int d = 0; d < 3; ++d) {
         if (c->dtiles[d]) hipFree(c->dtiles[d]);
+        if (c->dstiles[d]) hipFree(c->dstiles[d]);
         if (c->drva[d]) hipFree(c->drva[d]);
         if (c->cmf[d]) hipFree(c->cmf[d]);
         if (c->surf[d]) hipFree(c->surf[d]);
@@ -1430,13 +1641,7 @@ int cnd_deriv_sweep(cnd_ctx *c, int dir, const double *const *in, double *const 
     int r;
     double *a = c->tau, *o = c->tau + c->NT;   // scratch fields
     if ((r = sweep_io(c, in, a))) return r;
-    if (c->precision == CND_MIXED) {
-        if ((r = mixed_fd_field(c->st, c->dblk, c->dtab, c->dlu, c->dtiles[dir], unsigned(c->htiles[dir].size()), dir, a, o,
-                                c->fscr, c->NT)))
-            return fail(c, r, "mixed sweep");
-    } else if ((r = run_fd_field(c, dir, 0, a, 0, o, 0, 1))) {
-        return r;
-    }
+    if ((r = run_fd_field(c, dir, 0, a, 0, o, 0, 1, c->precision == CND_MIXED))) return r;
     return sweep_out(c, o, out);
 }
 
@@ -1446,13 +1651,7 @@ int cnd_filter_sweep(cnd_ctx *c, int dir, double *const *f)
     int r;
     double *a = c->tau, *o = c->tau + c->NT;
     if ((r = sweep_io(c, (const double *const *)f, a))) return r;
-    if (c->precision == CND_MIXED) {
-        if ((r = mixed_filter_one(c->st, c->dblk, c->dtab, c->dlu, c->dtiles[dir], unsigned(c->htiles[dir].size()), dir, a, o,
-                                  c->fscr, c->NT)))
-            return fail(c, r, "mixed sweep");
-    } else if ((r = run_filter_field(c, dir, a, o, 1))) {
-        return r;
-    }
+    if ((r = run_filter_field(c, dir, a, o, 1, c->precision == CND_MIXED))) return r;
     return sweep_out(c, o, f);
 }
 

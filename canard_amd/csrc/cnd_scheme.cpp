@@ -223,25 +223,47 @@ void build_tables(double fltk_over_pi, double fltrbc, Tables &t)
 // LU factorisation of the pentadiagonal LHS (src/mo_numerics.f90:179-249):
 // rows (xl1, xl2) are the lower factors, (xu1, xu2, xu3) the reciprocal
 // pivot and the two scaled upper entries.
+// LHS rows 0..2 next to a line end (offsets -2..2 from the row), for the
+// derivative (scheme 0: one-sided rows of mo_parameters.f90:66-79, or the
+// truncated interior rows at an interface) or the filter (scheme 1).
+static void end_rows(const Tables &t, int scheme, bool qd, double (*r)[5])
+{
+    if (scheme == 0) {
+        const double one_sided[3][5] = {{0, 0, 1, kAl01, kBe02}, {0, kAl10, 1, kAl12, kBe13},
+                                        {kBe, kAl, 1, kAl, kBe}};
+        const double qd_rows[3][5] = {{0, 0, 1, kAl, kBe}, {0, kAl, 1, kAl, kBe}, {kBe, kAl, 1, kAl, kBe}};
+        std::memcpy(r, qd ? qd_rows : one_sided, sizeof(double) * 15);
+    } else {
+        const double qd_rows[3][5] = {{0, 0, 1, t.alf, t.bef}, {0, t.alf, 1, t.alf, t.bef},
+                                      {t.bef, t.alf, 1, t.alf, t.bef}};
+        if (qd) std::memcpy(r, qd_rows, sizeof(double) * 15);
+        else std::memcpy(r, t.flhs, sizeof(double) * 15);
+    }
+}
+
+void lhs_rows(const Tables &t, int scheme, int n, bool qd_lo, bool qd_hi, double *o)
+{
+    double lo[3][5], hi[3][5];
+    end_rows(t, scheme, qd_lo, lo);
+    end_rows(t, scheme, qd_hi, hi);
+    const double al = scheme == 0 ? kAl : t.alf, be = scheme == 0 ? kBe : t.bef;
+    for (int i = 0; i < n; ++i) {
+        double *r = o + 5 * i;
+        r[0] = be; r[1] = al; r[2] = 1.0; r[3] = al; r[4] = be;
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int c = 0; c < 5; ++c) {
+            o[5 * i + c] = lo[i][c];
+            o[5 * (n - 1 - i) + c] = hi[i][4 - c];   // the far end mirrors the near-end rows
+        }
+}
+
 void lu_factors(const Tables &t, int scheme, int n, bool qd_lo, bool qd_hi, double *o)
 {
     // LHS rows 0..2 for the end family (offsets -2..2)
     double lo[3][5], hi[3][5], al, be;
-    auto set_rows = [&](bool qd, double (*r)[5]) {
-        if (scheme == 0) {
-            const double one_sided[3][5] = {{0, 0, 1, kAl01, kBe02}, {0, kAl10, 1, kAl12, kBe13},
-                                            {kBe, kAl, 1, kAl, kBe}};
-            const double qd_rows[3][5] = {{0, 0, 1, kAl, kBe}, {0, kAl, 1, kAl, kBe}, {kBe, kAl, 1, kAl, kBe}};
-            std::memcpy(r, qd ? qd_rows : one_sided, sizeof(double) * 15);
-        } else {
-            const double qd_rows[3][5] = {{0, 0, 1, t.alf, t.bef}, {0, t.alf, 1, t.alf, t.bef},
-                                          {t.bef, t.alf, 1, t.alf, t.bef}};
-            if (qd) std::memcpy(r, qd_rows, sizeof(double) * 15);
-            else std::memcpy(r, t.flhs, sizeof(double) * 15);
-        }
-    };
-    set_rows(qd_lo, lo);
-    set_rows(qd_hi, hi);
+    end_rows(t, scheme, qd_lo, lo);
+    end_rows(t, scheme, qd_hi, hi);
     if (scheme == 0) { al = kAl; be = kBe; } else { al = t.alf; be = t.bef; }
     auto L1 = [&](int i) -> double & { return o[5 * i + 0]; };
     auto L2 = [&](int i) -> double & { return o[5 * i + 1]; };
@@ -284,6 +306,7 @@ bool check_topology(const std::vector<BlockSpec> &blocks, std::string &err)
         const BlockSpec &B = blocks[b];
         for (int d = 0; d < 3; ++d) {
             if (B.n[d] < kQd + 1) { err = "block " + std::to_string(b) + ": every direction needs >= 13 points"; return false; }
+            if (B.n[d] > 2112) { err = "block " + std::to_string(b) + ": at most 2112 points per direction"; return false; }
             for (int s = 0; s < 2; ++s) {
                 const int c = B.bc[d][s];
                 const bool iface = c == kInterCurv || c == kInterStraight || c == kInterSub || c == kPeriodic;

@@ -43,6 +43,43 @@ void build_tables(double fltk_over_pi, double fltrbc, Tables &t);
 // 1 = filter; qd_lo/qd_hi select the quasi-disjoint rows at each end.
 // Output rows: (xl1, xl2, xu1, xu2, xu3) per point.
 void lu_factors(const Tables &t, int scheme, int n, bool qd_lo, bool qd_hi, double *rows5);
+// The banded LHS itself, 5 entries per row at offsets -2..2 (diagonal 1).
+void lhs_rows(const Tables &t, int scheme, int n, bool qd_lo, bool qd_hi, double *rows5);
+
+// Homogeneous solutions of the LU recurrences of one line (rows5 as from
+// lu_factors) restarted at every segment of an S-segment partition: per row
+// (phi1, phi2, psi1, psi2).  phi solves the forward recurrence of the
+// segment with incoming (y[a-2], y[a-1]) = (1, 0) / (0, 1); psi solves the
+// backward recurrence with incoming (x[b], x[b+1]) = (1, 0) / (0, 1).
+// bounds(s, a, len) gives segment s.
+template <class Bounds>
+void seg_tables(const double *rows5, int S, Bounds bounds, double *out4)
+{
+    for (int s = 0; s < S; ++s) {
+        int a, len;
+        bounds(s, a, len);
+        double p1a = 1.0, p1b = 0.0, p2a = 0.0, p2b = 1.0;   // (j-2, j-1)
+        for (int j = a; j < a + len; ++j) {
+            const double *l = rows5 + 5 * j;
+            const double y1 = (0.0 - l[0] * p1a) - l[1] * p1b;
+            const double y2 = (0.0 - l[0] * p2a) - l[1] * p2b;
+            out4[4 * j] = y1;
+            out4[4 * j + 1] = y2;
+            p1a = p1b; p1b = y1;
+            p2a = p2b; p2b = y2;
+        }
+        double q1a = 1.0, q1b = 0.0, q2a = 0.0, q2b = 1.0;   // (j+1, j+2)
+        for (int j = a + len - 1; j >= a; --j) {
+            const double *l = rows5 + 5 * j;
+            const double x1 = (l[2] * 0.0 - l[3] * q1a) - l[4] * q1b;
+            const double x2 = (l[2] * 0.0 - l[3] * q2a) - l[4] * q2b;
+            out4[4 * j + 2] = x1;
+            out4[4 * j + 3] = x2;
+            q1b = q1a; q1a = x1;
+            q2b = q2a; q2a = x2;
+        }
+    }
+}
 
 // Dense inverse by Gauss-Jordan elimination with row pivoting (n x n,
 // column-major).

@@ -1,0 +1,527 @@
+// Segmented pencil sweeps: the pentadiagonal solves of the compact
+// derivative (src/mo_numerics.f90:709-886) and compact filter
+// (src/mo_numerics.f90:890-987) with every pencil split into S segments of
+// at most L points, one lane per (pencil, segment, variable), so a lane keeps
+// its whole segment in registers and the sweep reads each source point once
+// and writes each result once (no forward-value scratch in HBM).
+//
+// The LU factors of the line (xl, xu of init_penta/penta) are the ones the
+// sequential sweep uses.  Each lane runs the forward and backward
+// recurrences over its own rows with zero incoming values; the true solution
+// follows by adding homogeneous solutions of the same recurrences restarted
+// at the segment start (forward: phi1, phi2) or end (backward: psi1, psi2),
+// weighted by the two values that enter from the neighbouring segment.
+// Those values are obtained by a short scan over the segments of the pencil
+// through LDS (a 2x2 transfer per segment).  The result equals the
+// sequential recurrence up to rounding.
+//
+// Workgroup: NV * P * SP threads.  Thread t: variable v = t / (P * SP),
+// pencil p = t % P, segment s = (t / P) % SP.  Sources are evaluated
+// cooperatively chunk by chunk (C rows of every segment) into LDS with
+// coalesced loads (xi: C consecutive points of one pencil per run; eta and
+// zeta: P neighbouring pencils per run) and each lane picks its variable.
+// The lanes holding a pencil end keep the 13 source values next to it in LDS;
+// the closure rows are formed from them after the stream (and, for a block
+// that is its own periodic partner, the partner sums too, so no extra loads).
+#pragma once
+
+#include "cnd_dev.hpp"
+#include "cnd_kernels.hpp"
+
+namespace cnd {
+
+// Points of a pencil end used by the closure rows (12) plus the face point.
+constexpr int kSegEnd = 12;
+constexpr int kSegEndRows = kSegEnd + 1;
+
+// Segments of an n-point pencil for lanes of at most L points: 0 when no
+// partition fits.  With S > 1 the last segment holds exactly the last L
+// points and the first n - L points are shared evenly by the others; the
+// first segment must hold the 12 points its closure rows use.
+__host__ __device__ inline int seg_count(int n, int L)
+{
+    if (n <= L) return 1;
+    const int S = (n + L - 1) / L;
+    if ((n - L) / (S - 1) < kSegEnd) return 0;
+    return S;
+}
+
+__host__ __device__ inline void seg_bounds(int n, int L, int S, int s, int &a, int &len)
+{
+    if (S == 1) { a = 0; len = n; return; }
+    if (s == S - 1) { a = n - L; len = L; return; }
+    const int m = n - L, q = m / (S - 1), r = m % (S - 1);
+    a = s * q + (s < r ? s : r);
+    len = q + (s < r ? 1 : 0);
+}
+
+template <int NV, int SCH, int L, int P, int C, bool XI>
+struct SegShape {
+    static constexpr int H = SCH == 0 ? 2 : 3;          // stencil half width
+    static constexpr int NB = SCH == 0 ? 2 : 3;         // closure rows per end
+    static constexpr int KR = L + 2 * H;                // rows streamed per lane
+    static constexpr int NCH = (KR + C - 1) / C;        // staging chunks
+    static constexpr int NOUT = (L + C - 1) / C;        // output chunks (xi)
+    // LDS: a work area (chunk buffer [C][NV][SP][P], then partner points
+    // [2][13][NV][P], then the exchange buffer [4][NV][SP][P]) and the
+    // pencil-end values [2][13][NV][P].
+    __host__ __device__ static constexpr int work_doubles(int SP)
+    {
+        const int chunk = C * NV * SP * P, part = 2 * kSegEndRows * NV * P, ex = 4 * NV * SP * P;
+        const int m = chunk > part ? chunk : part;
+        return m > ex ? m : ex;
+    }
+    __host__ __device__ static constexpr size_t lds_bytes(int SP)
+    {
+        return sizeof(double) * (size_t(work_doubles(SP)) + size_t(2) * kSegEndRows * NV * P);
+    }
+};
+
+// Chunk slot -> (o, s, p): xi runs along the pencil, eta/zeta across pencils.
+template <int P, int C, bool XI>
+__device__ __forceinline__ void seg_slot(int slot, int SP, int &o, int &s, int &p)
+{
+    if (XI) { o = slot % C; p = (slot / C) % P; s = slot / (C * P); }
+    else { p = slot % P; s = (slot / P) % SP; o = slot / (P * SP); }
+}
+
+// One lane's view of its segment.
+struct SegLane {
+    int a, len, s, S, n, L, ok, exi, exs, v, p, P, SP;
+};
+
+// Solve the line for the lane's segment held in Y (right-hand side in,
+// solution out) with LU factors LU and homogeneous solutions ST of type T;
+// ex is LDS room for 2 x NV x SP x P doubles.  All threads must call it.
+template <class T, int L>
+__device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const T *__restrict__ LU,
+                                          const T *__restrict__ ST, double *ex)
+{
+    const int a = ln.a, len = ln.len, s = ln.s;
+    // forward recurrence with zero incoming values
+    {
+        T ya = T(0), yb = T(0);
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            if (k < len) {
+                const T *lr = LU + 5 * (a + k);
+                const T y = (Y[k] - lr[0] * ya) - lr[1] * yb;
+                Y[k] = y;
+                ya = yb;
+                yb = y;
+            }
+        }
+        __syncthreads();
+        ex[ln.exi] = double(ya);
+        ex[ln.exs + ln.exi] = double(yb);
+        __syncthreads();
+    }
+    // incoming values of the segment: scan over the segments before it
+    if (ln.ok && s > 0) {
+        T t0 = T(0), t1 = T(0);
+        for (int sp = 0; sp < s; ++sp) {
+            int sa, sl;
+            seg_bounds(ln.n, ln.L, ln.S, sp, sa, sl);
+            const T *m0 = ST + 4 * (sa + sl - 2), *m1 = ST + 4 * (sa + sl - 1);
+            const int xi = ln.exi - (s - sp) * ln.P;
+            const T n0 = (m0[0] * t0 + m0[1] * t1) + T(ex[xi]);
+            const T n1 = (m1[0] * t0 + m1[1] * t1) + T(ex[ln.exs + xi]);
+            t0 = n0;
+            t1 = n1;
+        }
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            if (k < len) {
+                const T *ph = ST + 4 * (a + k);
+                Y[k] = Y[k] + (ph[0] * t0 + ph[1] * t1);
+            }
+        }
+    }
+    // backward recurrence with zero incoming values
+    {
+        T xa = T(0), xb = T(0);
+#pragma unroll
+        for (int k = L - 1; k >= 0; --k) {
+            if (k < len) {
+                const T *lr = LU + 5 * (a + k);
+                const T x = (lr[2] * Y[k] - lr[3] * xa) - lr[4] * xb;
+                Y[k] = x;
+                xb = xa;
+                xa = x;
+            }
+        }
+        __syncthreads();
+        ex[ln.exi] = double(xa);
+        ex[ln.exs + ln.exi] = double(xb);
+        __syncthreads();
+    }
+    // incoming values from above: scan over the segments after it
+    if (ln.ok && s < ln.S - 1) {
+        T h0 = T(0), h1 = T(0);
+        for (int sp = ln.S - 1; sp > s; --sp) {
+            int sa, sl;
+            seg_bounds(ln.n, ln.L, ln.S, sp, sa, sl);
+            const T *m0 = ST + 4 * sa, *m1 = ST + 4 * (sa + 1);
+            const int xi = ln.exi + (sp - s) * ln.P;
+            const T n0 = (m0[2] * h0 + m0[3] * h1) + T(ex[xi]);
+            const T n1 = (m1[2] * h0 + m1[3] * h1) + T(ex[ln.exs + xi]);
+            h0 = n0;
+            h1 = n1;
+        }
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            if (k < len) {
+                const T *ps = ST + 4 * (a + k);
+                Y[k] = Y[k] + (ps[2] * h0 + ps[3] * h1);
+            }
+        }
+    }
+}
+
+// Coefficient tables of the segmented sweeps, indexed like the LU rows:
+// LU factors and homogeneous solutions in FP64 and FP32, and the banded LHS
+// (for the FP64 residual of the mixed-precision solve).
+struct SegTabs {
+    const double *lu, *seg, *lhs;
+    const float *lu32, *seg32;
+};
+
+// One tile of P pencils of block tile.b along d.
+//  Src: load(g, b, Raw&), finish(Raw, b, double (&)[NV])
+//  Sink: store(g, b, const double (&)[NV]), put(g, b, v, x), ends(side, pencil, v, x)
+// MIX: solve in FP32 and refine once with the FP64 residual r = b - A x
+// (A the banded LHS), x += A^-1 r solved in FP32.
+template <int NV, int SCH, int L, int P, int C, bool XI, bool MIX, class Src, class Sink>
+__device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
+                                         const SegTabs &tb, Tile tile, int d, int n45, int SP, const Src &src,
+                                         Sink &sink, double *lds)
+{
+    using Sh = SegShape<NV, SCH, L, P, C, XI>;
+    constexpr int H = Sh::H;
+    const int nthr = NV * P * SP;
+    const int t = threadIdx.x;
+    const int v = t / (P * SP), p = t % P, s = (t / P) % SP;
+    const int b = tile.b;
+    const BlkDev &B = blks[b];
+    const TabDev &T = tabs[B.inst];
+    const int npen = npencils(B, d);
+    const int n = B.n[d], N = n - 1;
+    const int S = seg_count(n, L);
+    const double *LU = tb.lu + 5 * int64_t(B.lu[SCH][d]);
+    const double *ST = tb.seg + 4 * int64_t(B.lu[SCH][d]);
+    double *buf = lds;                                  // work area
+    double *ends = lds + Sh::work_doubles(SP);         // [2][13][NV][P]
+    const bool pen_ok = tile.q0 + p < npen;
+    const bool lane_ok = pen_ok && s < S;
+    int a = 0, len = 0;
+    if (s < S) seg_bounds(n, L, S, s, a, len);
+    auto pt = [&](int pp, int i) -> int64_t {
+        int64_t pb, st;
+        pencil_of(B, d, tile.q0 + pp, pb, st);
+        return pb + int64_t(i) * st;
+    };
+    auto eidx = [&](int side, int i, int m, int pp) { return ((side * kSegEndRows + i) * NV + m) * P + pp; };
+
+    // ---- 1. stream the segment (with H halo points each side): interior
+    //         right-hand sides into registers, end values into LDS
+    double Y[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) Y[k] = 0.0;
+    double w[2 * H + 1];
+#pragma unroll
+    for (int m = 0; m <= 2 * H; ++m) w[m] = 0.0;
+    const int jfirst = SCH == 0 ? 2 : 3, jlast = SCH == 0 ? N - 2 : N - 3;
+    const int nslots = C * SP * P;
+#pragma unroll
+    for (int c = 0; c < Sh::NCH; ++c) {
+        __syncthreads();
+        for (int slot = t; slot < nslots; slot += nthr) {
+            int o, ss, pp;
+            seg_slot<P, C, XI>(slot, SP, o, ss, pp);
+            if (ss >= S || tile.q0 + pp >= npen) continue;
+            int sa, sl;
+            seg_bounds(n, L, S, ss, sa, sl);
+            const int row = sa - H + c * C + o;
+            if (row < 0 || row > N || row >= sa + sl + H) continue;
+            typename Src::Raw raw;
+            src.load(pt(pp, row), b, raw);
+            double f[NV];
+            src.finish(raw, b, f);
+#pragma unroll
+            for (int m = 0; m < NV; ++m) buf[((o * NV + m) * SP + ss) * P + pp] = f[m];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < C; ++o) {
+            const int kr = c * C + o;            // streamed point a - H + kr
+            if (kr >= Sh::KR) break;
+            const double fv = buf[((o * NV + v) * SP + s) * P + p];
+            const int row = a - H + kr;
+            if (lane_ok && row >= 0 && row <= N) {
+                if (s == 0 && row <= kSegEnd) ends[eidx(0, row, v, p)] = fv;
+                if (s == S - 1 && N - row <= kSegEnd) ends[eidx(1, N - row, v, p)] = fv;
+            }
+#pragma unroll
+            for (int m = 0; m < 2 * H; ++m) w[m] = w[m + 1];
+            w[2 * H] = fv;
+            const int k = kr - 2 * H;            // row a + k has its stencil complete
+            if (k < 0 || k >= L) continue;
+            const int j = a + k;
+            if (k < len && j >= jfirst && j <= jlast) {
+                double rr;
+                if constexpr (SCH == 0) {
+                    rr = kAa * (w[3] - w[1]) + kAb * (w[4] - w[0]);
+                } else {
+                    const double resk = w[3] + w[3];
+                    rr = (T.fa * ((w[2] + w[4]) - resk) + T.fb * ((w[1] + w[5]) - resk)) +
+                         T.fc * ((w[0] + w[6]) - resk);
+                }
+                Y[k] = rr;
+            }
+        }
+    }
+
+    // ---- 2. partner points of interfaces with another block (one round) ------
+    bool self[2];
+    for (int side = 0; side < 2; ++side) self[side] = B.qd[d][side] != 0 && B.part[d][side] == b;
+    const bool stage_part = (B.qd[d][0] != 0 && !self[0]) || (B.qd[d][1] != 0 && !self[1]);
+    __syncthreads();
+    if (stage_part) {
+        for (int slot = t; slot < 2 * kSegEndRows * P; slot += nthr) {
+            const int pp = slot % P, o = (slot / P) % kSegEndRows, side = slot / (P * kSegEndRows);
+            if (tile.q0 + pp >= npen || B.qd[d][side] == 0 || self[side]) continue;
+            const int Pb = B.part[d][side];
+            const BlkDev &BP = blks[Pb];
+            const int ps = 1 - side, code = BP.bc[d][ps];
+            const int iik = code == 40 ? 0 : 1;
+            const int NP = BP.n[d] - 1;
+            const int idx = o == kSegEnd ? (ps == 0 ? 0 : NP) : (ps == 0 ? o + iik : NP - (o + iik));
+            int64_t pb, pst;
+            pencil_of(BP, d, tile.q0 + pp, pb, pst);
+            typename Src::Raw raw;
+            src.load(pb + int64_t(idx) * pst, Pb, raw);
+            double f[NV];
+            src.finish(raw, Pb, f);
+#pragma unroll
+            for (int m = 0; m < NV; ++m) buf[eidx(side, o, m, pp)] = f[m];
+        }
+        __syncthreads();
+    }
+
+    // ---- 3. closure rows at both ends (same arithmetic as the sequential sweep)
+    double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        if (!(lane_ok && s == (side == 0 ? 0 : S - 1))) continue;
+        auto f = [&](int i) { return ends[eidx(side, i, v, p)]; };   // distance i from this end
+        const bool qd = B.qd[d][side] != 0;
+        double rows[3] = {0.0, 0.0, 0.0};
+        if (qd) {
+            const int Pb = B.part[d][side];
+            const int ps = 1 - side, code = blks[Pb].bc[d][ps];
+            const int iik = code == 40 ? 0 : 1;
+            const double ra0k = code == 45 ? double(n45) : 0.0;
+            // partner point o (o = 12: the partner's face point)
+            auto g = [&](int o) {
+                if (self[side]) return ends[eidx(ps, o == kSegEnd ? 0 : o + iik, v, p)];
+                return buf[eidx(side, o, v, p)];
+            };
+            double r0 = 0.0, r1 = 0.0, sap0 = 0.0;
+            for (int i = 0; i < kSegEnd; ++i) {
+                const double fp = g(i);
+                if (i == 0) sap0 = fp;
+                r0 = r0 + T.qout[SCH][0][i] * fp;
+                r1 = r1 + T.qout[SCH][1][i] * fp;
+            }
+            const double resk = ra0k != 0.0 ? ra0k * g(kSegEnd) : 0.0;
+            r0 = r0 - resk * T.qsum[SCH][0];
+            r1 = r1 - resk * T.qsum[SCH][1];
+            double r2 = sap0 - resk;
+            if (n45 == 1 && B.bc[d][side] == 45) {
+                const double fo = f(0);
+                r0 = r0 + fo * T.qsum[SCH][0];
+                r1 = r1 + fo * T.qsum[SCH][1];
+                r2 = r2 + double(SCH) * fo;
+            }
+            double a0 = 0.0, a1 = 0.0;
+            for (int i = 0; i < kSegEnd; ++i) {
+                a0 = a0 + T.qin[SCH][0][i] * f(i);
+                a1 = a1 + T.qin[SCH][1][i] * f(i);
+            }
+            if (SCH == 0) {
+                rows[0] = side == 0 ? a0 + r0 : -a0 - r0;
+                rows[1] = side == 0 ? a1 + r1 : -a1 - r1;
+            } else {
+                rows[0] = a0 + r0;
+                rows[1] = a1 + r1;
+                const double ra2k = f(2) + f(2);
+                if (side == 0)
+                    rows[2] = (T.fa * ((f(1) + f(3)) - ra2k) + T.fb * ((f(0) + f(4)) - ra2k)) +
+                              T.fc * ((r2 + f(5)) - ra2k);
+                else
+                    rows[2] = (T.fa * ((f(3) + f(1)) - ra2k) + T.fb * ((f(4) + f(0)) - ra2k)) +
+                              T.fc * ((f(5) + r2) - ra2k);
+            }
+        } else if (SCH == 0) {
+            double sum = 0.0;
+            if (side == 0) {
+                sum = sum + kA01 * (f(1) - f(0)); sum = sum + kA02 * (f(2) - f(0));
+                sum = sum + kA03 * (f(3) - f(0)); sum = sum + kA04 * (f(4) - f(0));
+                rows[0] = sum;
+                sum = 0.0;
+                sum = sum + kA10 * (f(0) - f(1)); sum = sum + kA12 * (f(2) - f(1));
+                sum = sum + kA13 * (f(3) - f(1)); sum = sum + kA14 * (f(4) - f(1));
+                rows[1] = sum;
+            } else {
+                sum = sum + kA01 * (f(0) - f(1)); sum = sum + kA02 * (f(0) - f(2));
+                sum = sum + kA03 * (f(0) - f(3)); sum = sum + kA04 * (f(0) - f(4));
+                rows[0] = sum;
+                sum = 0.0;
+                sum = sum + kA10 * (f(1) - f(0)); sum = sum + kA12 * (f(1) - f(2));
+                sum = sum + kA13 * (f(1) - f(3)); sum = sum + kA14 * (f(1) - f(4));
+                rows[1] = sum;
+            }
+        } else {
+            const int o0[5] = {1, 2, 3, 4, 5}, o1[5] = {0, 2, 3, 4, 5}, o2[5] = {0, 1, 3, 4, 5};
+            double sum = 0.0;
+            for (int cc = 0; cc < 5; ++cc) sum = sum + T.fbc[0][cc] * (f(o0[cc]) - f(0));
+            rows[0] = sum;
+            sum = 0.0;
+            for (int cc = 0; cc < 5; ++cc) sum = sum + T.fbc[1][cc] * (f(o1[cc]) - f(1));
+            rows[1] = sum;
+            sum = 0.0;
+            for (int cc = 0; cc < 5; ++cc) sum = sum + T.fbc[2][cc] * (f(o2[cc]) - f(2));
+            rows[2] = sum;
+        }
+        for (int r = 0; r < 3; ++r) (side == 0 ? lo : hi)[r] = rows[r];
+    }
+    // closure rows: rows 0..NB-1 open segment 0; rows N-r close the last one
+    if (lane_ok && s == 0) {
+#pragma unroll
+        for (int r = 0; r < Sh::NB; ++r) Y[r] = lo[r];
+    }
+    if (lane_ok && s == S - 1) {
+        if (S > 1) {
+#pragma unroll
+            for (int r = 0; r < Sh::NB; ++r) Y[L - 1 - r] = hi[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < Sh::NB; ++r) {
+                const int kk = N - r;
+#pragma unroll
+                for (int k = 0; k < L; ++k)
+                    if (k == kk) Y[k] = hi[r];
+            }
+        }
+    }
+
+    // ---- 4. solve: FP64, or FP32 with one FP64 residual refinement ----------
+    double *ex = buf;
+    const SegLane ln{a, len, s, S, n, L, lane_ok, (v * SP + s) * P + p, NV * SP * P, v, p, P, SP};
+    if constexpr (!MIX) {
+        seg_solve<double, L>(Y, ln, LU, ST, ex);
+    } else {
+        float X[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) X[k] = float(Y[k]);
+        seg_solve<float, L>(X, ln, tb.lu32 + 5 * int64_t(B.lu[SCH][d]), tb.seg32 + 4 * int64_t(B.lu[SCH][d]), ex);
+        // edge values of the neighbouring segments: x(a-2), x(a-1), x(b), x(b+1)
+        double l2 = 0.0, l1 = 0.0, r1 = 0.0, r2 = 0.0;
+        {
+            double e0 = 0.0, e1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < L; ++k) {
+                if (k == len - 2) e0 = X[k];
+                if (k == len - 1) e1 = X[k];
+            }
+            __syncthreads();
+            ex[ln.exi] = double(X[0]);
+            ex[ln.exs + ln.exi] = double(X[1]);
+            ex[2 * ln.exs + ln.exi] = e0;
+            ex[3 * ln.exs + ln.exi] = e1;
+            __syncthreads();
+            if (lane_ok && s > 0) {
+                const int xi = ln.exi - P;
+                l2 = ex[2 * ln.exs + xi];
+                l1 = ex[3 * ln.exs + xi];
+            }
+            if (lane_ok && s < S - 1) {
+                const int xi = ln.exi + P;
+                r1 = ex[xi];
+                r2 = ex[ln.exs + xi];
+            }
+        }
+        // r = b - A x in FP64 with the banded rows of the line
+        const double *AR = tb.lhs + 5 * int64_t(B.lu[SCH][d]);
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            if (k < len) {
+                const double *ar = AR + 5 * (a + k);
+                const double xm2 = k >= 2 ? double(X[k >= 2 ? k - 2 : 0]) : (k == 1 ? l1 : l2);
+                const double xm1 = k >= 1 ? double(X[k >= 1 ? k - 1 : 0]) : l1;
+                const double xp1 = k + 1 < len ? double(X[k + 1 < L ? k + 1 : L - 1]) : r1;
+                const double xp2 = k + 2 < len ? double(X[k + 2 < L ? k + 2 : L - 1]) : (k + 1 < len ? r1 : r2);
+                const double ax = (((ar[0] * xm2 + ar[1] * xm1) + double(X[k])) + ar[3] * xp1) + ar[4] * xp2;
+                Y[k] = Y[k] - ax;
+            }
+        }
+        float D[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) D[k] = float(Y[k]);
+        seg_solve<float, L>(D, ln, tb.lu32 + 5 * int64_t(B.lu[SCH][d]), tb.seg32 + 4 * int64_t(B.lu[SCH][d]), ex);
+#pragma unroll
+        for (int k = 0; k < L; ++k) Y[k] = double(X[k]) + double(D[k]);
+    }
+    if (lane_ok && s == 0) sink.ends(0, p, v, Y[0]);
+    if (lane_ok && s == S - 1) {
+        double xn = 0.0;
+        if (S > 1) {
+            xn = Y[L - 1];
+        } else {
+#pragma unroll
+            for (int k = 0; k < L; ++k)
+                if (k == N) xn = Y[k];
+        }
+        sink.ends(1, p, v, xn);
+    }
+
+    // ---- 6. results ------------------------------------------------------------
+    if constexpr (!XI) {
+        // eta/zeta: lanes of a wave hold P neighbouring pencils, stores coalesce
+        if (lane_ok) {
+            int64_t pb, st;
+            pencil_of(B, d, tile.q0 + p, pb, st);
+            pb += int64_t(a) * st;
+#pragma unroll
+            for (int k = 0; k < L; ++k)
+                if (k < len) sink.put(pb + int64_t(k) * st, b, v, Y[k]);
+        }
+    } else {
+        // xi: through LDS, stored in runs along the pencil
+#pragma unroll
+        for (int c = 0; c < Sh::NOUT; ++c) {
+            __syncthreads();
+#pragma unroll
+            for (int o = 0; o < C; ++o) {
+                const int k = c * C + o;
+                if (k < L && k < len) buf[((o * NV + v) * SP + s) * P + p] = Y[k];
+            }
+            __syncthreads();
+            for (int slot = t; slot < nslots; slot += nthr) {
+                int o, ss, pp;
+                seg_slot<P, C, XI>(slot, SP, o, ss, pp);
+                if (ss >= S || tile.q0 + pp >= npen) continue;
+                int sa, sl;
+                seg_bounds(n, L, S, ss, sa, sl);
+                const int k = c * C + o;
+                if (k >= sl) continue;
+                double x[NV];
+#pragma unroll
+                for (int m = 0; m < NV; ++m) x[m] = buf[((o * NV + m) * SP + ss) * P + pp];
+                sink.store(pt(pp, sa + k), b, x);
+            }
+        }
+    }
+}
+
+}  // namespace cnd

@@ -11,7 +11,7 @@
 //      or the flux components) at the 12 points next to each pencil end and
 //      at the partner pencil's 12 points, stages them in LDS, and each wave
 //      forms its variable's boundary rows in the reference's order;
-//   2. the pencil is streamed in chunks of kChunk points: the workgroup
+//   2. the pencil is streamed in chunks of C points (chunk_of): the workgroup
 //      evaluates the source for chunk x 64 points with coalesced loads (xi:
 //      runs of consecutive points of one pencil; eta/zeta: one point of 64
 //      neighbouring pencils) into LDS, and each wave advances its forward
@@ -26,8 +26,19 @@
 namespace cnd {
 
 constexpr int kTile = 64;    // pencils per tile (one wave per variable)
-constexpr int kChunk = 8;    // points per chunk along the pencil
 constexpr int kEnd = 12;     // points staged per pencil end
+
+// Points per chunk along the pencil.  xi pencils are contiguous, so a chunk
+// reads chunk x 8 bytes of each pencil: 16 points fill a 128-byte cache
+// line; eta/zeta chunks read whole 64-pencil rows and stay at 8 points.
+#ifndef CND_CHUNK_XI
+#define CND_CHUNK_XI 16
+#endif
+#ifndef CND_CHUNK_ETA
+#define CND_CHUNK_ETA 8
+#endif
+constexpr int kChunkXi = CND_CHUNK_XI, kChunkEta = CND_CHUNK_ETA;
+__host__ __device__ constexpr int chunk_of(int d) { return d == 0 ? kChunkXi : kChunkEta; }
 
 // LDS block [o][v][p], one double of padding per o-row.
 template <int NV>
@@ -37,8 +48,19 @@ struct LdsRows {
     __device__ double &at(int o, int v, int p) const { return s[o * kPitch + v * kTile + p]; }
 };
 
-template <int NV>
-constexpr size_t sweep_lds_bytes() { return sizeof(double) * kEnd * (NV * kTile + 1); }
+template <int NV, int C>
+constexpr size_t sweep_lds_bytes() { return sizeof(double) * (C > kEnd ? C : kEnd) * (NV * kTile + 1); }
+
+// Workgroups are handed to the 8 XCDs (each with its own L2) round robin.
+// Workgroup w takes tile (w % 8) * per + w / 8, so consecutive tiles, which
+// share the cache lines at their edges, run on one XCD at about the same
+// time.  The grid is 8 * per wide; workgroups past the last tile return.
+__host__ __device__ inline int xcd_width(int ntiles) { return 8 * ((ntiles + 7) / 8); }
+__device__ inline int xcd_tile(int ntiles)
+{
+    const int per = (ntiles + 7) / 8;
+    return int(blockIdx.x % 8) * per + int(blockIdx.x / 8);
+}
 
 // Staging slot k of thread t: which pencil p of the tile and which offset o
 // of an nrows-point window.  xi pencils are contiguous, so consecutive
@@ -57,8 +79,9 @@ __device__ __forceinline__ int64_t yidx(int64_t tile, int NV, int nmax, int v, i
 }
 
 // Evaluate the source for all 64 pencils at nrows points each and stage
-// them: point index of (pencil p, offset o) given by pt(p, o); pencil_base
-// gives (block, base, stride) of pencil p.
+// them.  where(p, o, g, blk) gives the point of (pencil p, offset o).
+// (Issuing all of a thread's loads before using any was measured slower:
+// the extra live registers cost more occupancy than the loads gain.)
 template <int NV, class Src, class Where>
 __device__ __forceinline__ void stage(int d, int nrows, const Src &src, const Where &where, LdsRows<NV> lds)
 {
@@ -69,8 +92,10 @@ __device__ __forceinline__ void stage(int d, int nrows, const Src &src, const Wh
         int64_t g;
         int blk;
         if (where(p, o, g, blk)) {
+            typename Src::Raw raw;
+            src.load(g, blk, raw);
             double v[NV];
-            src.eval(g, blk, v);
+            src.finish(raw, blk, v);
 #pragma unroll
             for (int k = 0; k < NV; ++k) lds.at(o, k, p) = v[k];
         }
@@ -81,7 +106,7 @@ __device__ __forceinline__ void stage(int d, int nrows, const Src &src, const Wh
 //  Src::eval(g, b, double (&v)[NV])            source values at point g
 //  Sink::store(g, b, const double (&x)[NV])     final values (cooperative)
 //  Sink::ends(side, lane, v, x)                 pencil end value of variable v
-template <int NV, int S, class Src, class Sink>
+template <int NV, int S, int C, class Src, class Sink>
 __device__ __forceinline__ void sweep_tile(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
                                            const double *__restrict__ lu, Tile tile, int64_t tile_id, int d,
                                            int n45, int nmax, double *__restrict__ ys, const Src &src, Sink &sink,
@@ -231,7 +256,7 @@ __device__ __forceinline__ void sweep_tile(const BlkDev *__restrict__ blks, cons
     }
 
     // ---- 2. forward substitution, chunk by chunk -----------------------------
-    const int nch = (n + kChunk - 1) / kChunk;
+    const int nch = (n + C - 1) / C;
     double w[2 * H + 1];     // window f(i-2H..i)
     double ya = 0.0, yb = 0.0;
     double *yrow = ys + yidx(tile_id, NV, nmax, wv, 0, lane);
@@ -249,8 +274,8 @@ __device__ __forceinline__ void sweep_tile(const BlkDev *__restrict__ blks, cons
     const int jfirst = S == 0 ? 2 : 3, jlast = S == 0 ? N - 2 : N - 3;
     for (int c = 0; c < nch; ++c) {
         __syncthreads();
-        stage<NV>(d, kChunk, src, [&](int p, int o, int64_t &g, int &blk) {
-            const int i = c * kChunk + o;
+        stage<NV>(d, C, src, [&](int p, int o, int64_t &g, int &blk) {
+            const int i = c * C + o;
             if (tile.q0 + p >= npen || i >= n) return false;
             g = own(p, i);
             blk = b;
@@ -259,8 +284,8 @@ __device__ __forceinline__ void sweep_tile(const BlkDev *__restrict__ blks, cons
         __syncthreads();
         if (!active) continue;
 #pragma unroll
-        for (int o = 0; o < kChunk; ++o) {
-            const int i = c * kChunk + o;
+        for (int o = 0; o < C; ++o) {
+            const int i = c * C + o;
             if (i >= n) break;
 #pragma unroll
             for (int m = 0; m < 2 * H; ++m) w[m] = w[m + 1];
@@ -268,7 +293,7 @@ __device__ __forceinline__ void sweep_tile(const BlkDev *__restrict__ blks, cons
             const int j = i - H;   // row whose stencil is now complete
             if (j < jfirst || j > jlast) continue;
             double rr;
-            if (S == 0) {
+            if constexpr (S == 0) {
                 rr = kAa * (w[3] - w[1]) + kAb * (w[4] - w[0]);
             } else {
                 const double resk = w[3] + w[3];
@@ -297,8 +322,8 @@ __device__ __forceinline__ void sweep_tile(const BlkDev *__restrict__ blks, cons
     for (int c = nch - 1; c >= 0; --c) {
         if (active) {
 #pragma unroll
-            for (int o = kChunk - 1; o >= 0; --o) {
-                const int j = c * kChunk + o;
+            for (int o = C - 1; o >= 0; --o) {
+                const int j = c * C + o;
                 if (j > N) continue;
                 double x;
                 if (j == N) {
@@ -315,10 +340,10 @@ __device__ __forceinline__ void sweep_tile(const BlkDev *__restrict__ blks, cons
             }
         }
         __syncthreads();
-        for (int slot = threadIdx.x; slot < kChunk * kTile; slot += NV * kTile) {
+        for (int slot = threadIdx.x; slot < C * kTile; slot += NV * kTile) {
             int p, o;
-            stage_slot(d, kChunk, slot, p, o);
-            const int j = c * kChunk + o;
+            stage_slot(d, C, slot, p, o);
+            const int j = c * C + o;
             if (tile.q0 + p < npen && j < n) {
                 double x[NV];
 #pragma unroll

@@ -34,21 +34,38 @@ def two_block_x(n=(20, 16), let=16, lze=12, code=35):
 
 SWEEP_CASES = {
     "periodic16": lambda: ([cases.periodic_block(16)], [cases.gridtoy(16, 16, 16)]),
+    # long pencils: several segments per pencil in every direction
+    "periodic_100x60x40": lambda: ([dict(lxi=100, let=60, lze=40, nbc=[[45, 45]] * 3, mcd=[[0, 0]] * 3)],
+                                   [cases.gridtoy(100, 60, 40)]),
+    "mixed_bc_80x70x50": lambda: ([dict(lxi=80, let=70, lze=50, nbc=[[10, 10], [45, 45], [25, 20]],
+                                        mcd=[[-1, -1], [0, 0], [-1, -1]])], [cases.gridtoy(80, 70, 50)]),
+    "two_block_60_50": lambda: two_block_x(n=(60, 50), let=40, lze=36),
     "mixed_bc": lambda: ([dict(lxi=24, let=20, lze=16, nbc=[[10, 10], [45, 45], [25, 20]],
                                mcd=[[-1, -1], [0, 0], [-1, -1]])], [cases.gridtoy(24, 20, 16)]),
     "two_block": lambda: two_block_x(),
 }
 
 
+@pytest.fixture(params=["segmented", "tiled"])
+def sweep_mode(request, monkeypatch):
+    """Both device sweep algorithms: register-resident segments (default) and
+    the tiled sweep with forward values in scratch (CANARD_SWEEP=tiled)."""
+    if request.param == "tiled":
+        monkeypatch.setenv("CANARD_SWEEP", "tiled")
+    else:
+        monkeypatch.delenv("CANARD_SWEEP", raising=False)
+    return request.param
+
+
 @pytest.mark.parametrize("name", list(SWEEP_CASES))
-def test_single_sweeps_match_oracle(name):
+def test_single_sweeps_match_oracle(name, sweep_mode):
     blocks, grids = SWEEP_CASES[name]()
     rng = np.random.default_rng(11)
     npt = [(b["lxi"] + 1) * (b["let"] + 1) * (b["lze"] + 1) for b in blocks]
     fields = [np.sin(g[0]) * np.cos(2 * g[1]) + 0.3 * rng.standard_normal(n) for g, n in zip(grids, npt)]
     states = [cases.uniform_state(n) for n in npt]
     params = dict(reoo=1000.0)
-    print(f"[sweep] case={name}", flush=True)
+    print(f"[sweep] case={name} mode={sweep_mode}", flush=True)
     c = _case(blocks, grids, states, params)
     o = oracle_for(blocks, grids, states, params, quirk_send2=0)
     for d in range(3):
@@ -83,7 +100,7 @@ def test_metrics_match_oracle():
     o.close()
 
 
-@pytest.mark.parametrize("n,steps,pert", [(16, 12, 0.0), (24, 12, 1e-3)])
+@pytest.mark.parametrize("n,steps,pert", [(16, 12, 0.0), (24, 12, 1e-3), (48, 6, 1e-3)])
 def test_tgv_steps_match_oracle(n, steps, pert):
     blocks, grids, states, params = tgv_case(n, pert=pert)
     print(f"[steps] tgv n={n} steps={steps}", flush=True)
@@ -116,9 +133,14 @@ def test_noncubic_periodic_matches_oracle():
     o.close()
 
 
-def test_multiblock_gcbc_sponge_matches_oracle():
+def test_multiblock_gcbc_sponge_matches_oracle(sweep_mode):
     """Two sheared blocks joined by a BC 35 interface, non-reflective outer
-    faces, periodic z, sponge, uniform Mach 0.5 flow with a pressure pulse."""
+    faces, periodic z, sponge, uniform Mach 0.5 flow with a pressure pulse.
+    The change from the initial state is ~1e-3 of the state, so its relative
+    error shows rounding differences magnified 1e3 times: the tiled sweep
+    follows the oracle's operation order (1e-11), the segmented sweep adds
+    homogeneous corrections in another order (1e-10)."""
+    tol_change = 1e-11 if sweep_mode == "tiled" else 1e-10
     sizes = [(24, 16, 14), (16, 16, 14)]
     blocks = cases.sheared_pair_blocks(sizes)
     grids = cases.sheared_pair_grids(sizes, h=1.0, amp=0.05)
@@ -133,7 +155,7 @@ def test_multiblock_gcbc_sponge_matches_oracle():
     for b in range(2):
         err = rel_err(c.state(b) - states[b], o.state(b) - states[b])
         print("  block", b, "rel err of the change", err)
-        assert np.all(err < 1e-11), err
+        assert np.all(err < tol_change), err
         err = rel_err(c.state(b), o.state(b))
         assert np.all(err < 1e-12), err
     c.close()
