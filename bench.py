@@ -126,6 +126,8 @@ def main():
     params = dict(fltk=0.49, fltrbc=0.9, reoo=1600.0 / 0.1, tempoo=300.0, cfl=0.8, viscous=1)
     prec = CND_FP64 if a.precision == "fp64" else CND_MIXED
 
+    tuning = []
+
     def run(precision):
         """W warm-up steps, then K timed steps; returns (seconds, section
         timers, clock, final state, stored points, cells)."""
@@ -136,6 +138,7 @@ def main():
         c.use_graph(bool(a.graph))
         c.step(a.warmup)
         c.synchronize()
+        tuning.append(c.tuning())
         print(f"[bench] warm-up done, timing {a.steps} steps", file=sys.stderr, flush=True)
         c.profile(True)
         t0 = time.perf_counter()
@@ -196,7 +199,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"taylor_green_{n}^3_periodic_viscous_rk4", "cells": cells,
                    "stored_points": points, "mach": 0.1, "re": 1600.0, "perturbation": 1e-3,
-                   "precision": a.precision, "graph": bool(a.graph)},
+                   "precision": a.precision, "graph": bool(a.graph),
+                   "tuning": {k: [r["shape"] for r in v] for k, v in tuning[0].items()}},
         "hbm_pct_step_model": 100.0 * STEP_BYTES_MODEL * points * a.steps / dt / HBM_PEAK,
         "roofline": roof,
         "sections_ms_per_step": {k: v[0] / a.steps for k, v in sections.items()},

@@ -14,6 +14,8 @@ LIBPATH = os.environ.get("CANARD_LIB") or os.path.join(HERE, "libcnd.so")
 CND_FP64, CND_MIXED = 0, 1
 # section timers (include/cnd.h CND_SEC_*)
 SECTIONS = ["filter", "tstep", "grad", "stress", "flux", "gcbc", "update", "avg"]
+# sweep kinds of the autotuner (include/cnd.h CND_KIND_*)
+KINDS = ["filter", "grad", "flux", "field"]
 
 
 class Params(C.Structure):
@@ -34,6 +36,7 @@ class Block(C.Structure):
 # (name, restype, argtypes) of every symbol include/cnd.h declares
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 SIGNATURES = [
     ("cnd_create", C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(Block), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("cnd_destroy", None, [C.c_void_p]),
@@ -55,6 +58,26 @@ SIGNATURES = [
     ("cnd_filter_sweep", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_dp)]),
     ("cnd_profile", C.c_int, [C.c_void_p, C.c_int]),
     ("cnd_profile_read", C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_longlong)]),
+    ("cnd_tune", C.c_int, [C.c_void_p]),
+    ("cnd_set_shape", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    ("cnd_shape_info", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),
+    ("cnd_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _dp]),
+    ("cnd_net_param_count", C.c_int, []),
+    ("cnd_net_create", C.c_int, [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
+    ("cnd_net_destroy", None, [C.c_void_p]),
+    ("cnd_net_error", C.c_char_p, [C.c_void_p]),
+    ("cnd_net_get_params", C.c_int, [C.c_void_p, _fp]),
+    ("cnd_net_set_params", C.c_int, [C.c_void_p, _fp]),
+    ("cnd_net_set_norm", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("cnd_net_add_pair", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    ("cnd_net_add_pair_host", C.c_int, [C.c_void_p, _fp, _fp]),
+    ("cnd_net_pairs", C.c_int, [C.c_void_p]),
+    ("cnd_net_train", C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_ulonglong, _dp]),
+    ("cnd_net_eval", C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+    ("cnd_net_apply", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("cnd_net_forward_host", C.c_int, [C.c_void_p, _fp, _fp]),
+    ("cnd_net_grad_host", C.c_int, [C.c_void_p, _fp, _fp, _fp, _fp]),
+    ("cnd_box_filter", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     ("cnd_stream", C.c_void_p, [C.c_void_p]),
     ("cnd_points", C.c_longlong, [C.c_void_p]),
     ("cnd_cells", C.c_longlong, [C.c_void_p]),

@@ -169,3 +169,46 @@ def ensemble(seed=7, count=32, sizes=(32, 48, 64, 96, 128), mach=(0.05, 0.3), re
         r = float(rng.uniform(*re))
         out.append(dict(n=n, mach=m, re=r))
     return out
+
+
+# ---- the benchmark configurations -------------------------------------------
+
+def config3(seed=20261202, sizes=((384, 256, 64), (192, 256, 64), (512, 128, 64), (128, 128, 64)), pulses=3):
+    """Four uneven sheared blocks as two independent pairs joined on their xi
+    faces (BC 35), non-reflective outer x/y faces, periodic z; uniform Mach 0.5
+    flow along x with seeded Gaussian pressure pulses (amplitude 1e-3, width 5
+    cells) and a sponge (szco 0.5).  Returns (blocks, grids, states, params)."""
+    rng = np.random.default_rng(seed)
+    blocks, grids, states = [], [], []
+    for p, pair in enumerate((sizes[:2], sizes[2:])):
+        blocks += sheared_pair_blocks(list(pair), first=2 * p)
+        g = sheared_pair_grids(list(pair), h=1.0, amp=0.05)
+        lx = pair[0][0] + pair[1][0]
+        centres = [(rng.uniform(0.2, 0.8) * lx, rng.uniform(0.2, 0.8) * pair[0][1], rng.uniform(0, pair[0][2]))
+                   for _ in range(pulses)]
+        grids += g
+        states += [pulse_state(x, mach=0.5, centres=centres, amp=1e-3, width=5.0) for x in g]
+    lx_max = max(sizes[0][0] + sizes[1][0], sizes[2][0] + sizes[3][0])
+    params = dict(reoo=1000.0, amach1=0.5, nsmf=1, cfl=0.8, viscous=1, sponge=1, szco=0.5,
+                  szth0=0.1 * lx_max, szth1=0.1 * lx_max, doml0=0.0, doml1=float(lx_max))
+    return blocks, grids, states, params
+
+
+def config4(seed=20261202, count=32, sizes=(32, 48, 64, 96, 128)):
+    """Ensemble of periodic Taylor-Green instances: n^3 cells with n drawn
+    from sizes, Mach uniform in [0.05, 0.3], Re uniform in [400, 3200], a 1e-3
+    velocity perturbation, all by seed.  Returns (blocks, grids, states,
+    params list, draws)."""
+    rng = np.random.default_rng(seed)
+    blocks, grids, states, params, draws = [], [], [], [], []
+    for i in range(count):
+        n = int(rng.choice(sizes))
+        mach = float(rng.uniform(0.05, 0.3))
+        re = float(rng.uniform(400.0, 3200.0))
+        xyz = gridtoy(n, n, n, doml0=0.0, doml1=2 * np.pi, domh=np.pi, span=2 * np.pi)
+        blocks.append(periodic_block(n, inst=i, me=i))
+        grids.append(xyz)
+        states.append(tgv_state(xyz, (n + 1,) * 3, mach=mach, pert=1e-3, seed=seed + i))
+        params.append(dict(fltk=0.49, fltrbc=0.9, reoo=re / mach, tempoo=300.0, cfl=0.8, viscous=1))
+        draws.append(dict(n=n, mach=mach, re=re))
+    return blocks, grids, states, params, draws

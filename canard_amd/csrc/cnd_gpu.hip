@@ -21,6 +21,7 @@
 
 #include "../../include/cnd.h"
 #include "cnd_dev.hpp"
+#include "cnd_internal.hpp"
 #include "cnd_kernels.hpp"
 #include "cnd_scheme.hpp"
 #include "cnd_seg.hpp"
@@ -228,15 +229,36 @@ __global__ __launch_bounds__(5 * kTile) void k_fd_flux(const BlkDev *__restrict_
 }
 
 // ---- segmented sweeps (cnd_seg.hpp) ---------------------------------------
-// xi: tiles of 4 pencils, segments of <= 17 points, 16-point staging runs
-// along the pencil; eta/zeta: tiles of 16 pencils, segments of <= 33 points,
-// staging runs across 16 neighbouring pencils.  SPMAX bounds the segments per
-// pencil (and so the longest pencil these shapes take).
-struct SegXi { static constexpr int L = 17, P = 4, C = 16, SPMAX = 16; static constexpr bool XI = true; };
-#ifndef CND_SEG_ETA_C
-#define CND_SEG_ETA_C 8
-#endif
-struct SegEta { static constexpr int L = 33, P = 16, C = CND_SEG_ETA_C, SPMAX = 8; static constexpr bool XI = false; };
+// Compiled shapes: L points per segment (lanes per pencil = segments =
+// ceil(n / L)), P pencils per workgroup, C-point staging runs, at most SPMAX
+// segments per pencil; XI shapes stage runs along the pencil (xi sweeps),
+// the others across P neighbouring pencils (eta, zeta sweeps).  The
+// autotuner picks one per sweep kind and direction.
+template <int L_, int P_, int C_, int SPMAX_, bool XI_>
+struct SegG { static constexpr int L = L_, P = P_, C = C_, SPMAX = SPMAX_; static constexpr bool XI = XI_; };
+struct ShapeInfo { int L, P, C, SPMAX; bool XI; };
+constexpr int kNumShapes = 7;
+constexpr ShapeInfo kShapes[kNumShapes] = {
+    {17, 4, 16, 16, true},    // 0: xi default (pencils up to 272 points)
+    {33, 4, 16, 8, true},     // 1
+    {33, 16, 8, 8, false},    // 2: eta/zeta default (up to 264 points)
+    {33, 16, 4, 8, false},    // 3
+    {17, 8, 8, 16, false},    // 4
+    {33, 4, 16, 16, true},    // 5: long xi pencils (up to 528 points)
+    {33, 8, 8, 16, false}};   // 6: long eta/zeta pencils (up to 528 points)
+using Sh0 = SegG<17, 4, 16, 16, true>;
+using Sh1 = SegG<33, 4, 16, 8, true>;
+using Sh2 = SegG<33, 16, 8, 8, false>;
+using Sh3 = SegG<33, 16, 4, 8, false>;
+using Sh4 = SegG<17, 8, 8, 16, false>;
+using Sh5 = SegG<33, 4, 16, 16, true>;
+using Sh6 = SegG<33, 8, 8, 16, false>;
+constexpr int kTiled = -1;   // shape index of the tiled sweep
+// pencils-per-workgroup values (tile lists) and segment lengths (tables)
+constexpr int kNumP = 3, kPs[kNumP] = {4, 8, 16};
+constexpr int kNumL = 2, kLs[kNumL] = {17, 33};
+__host__ __device__ constexpr int p_index(int P) { return P == 4 ? 0 : (P == 8 ? 1 : 2); }
+__host__ __device__ constexpr int l_index(int L) { return L == 17 ? 0 : 1; }
 
 #define SEG_PROLOGUE(ntiles)               \
     const int tix = xcd_tile(ntiles);      \
@@ -246,7 +268,7 @@ struct SegEta { static constexpr int L = 33, P = 16, C = CND_SEG_ETA_C, SPMAX = 
 template <class G, bool MIX>
 __global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_fd_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int d, int n45, int SP, const double *in, int64_t in_vstride, double *out, int64_t out_vstride)
+    int ntiles, int SP, int d, int n45, const double *in, int64_t in_vstride, double *out, int64_t out_vstride)
 {
     SEG_PROLOGUE(ntiles)
     const int v = blockIdx.y;
@@ -258,7 +280,7 @@ __global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_fd_field(
 template <class G, bool MIX>
 __global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_filter_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int d, int SP, const double *in, double *out, int64_t vstride)
+    int ntiles, int SP, int d, const double *in, double *out, int64_t vstride)
 {
     SEG_PROLOGUE(ntiles)
     const int v = blockIdx.y;
@@ -272,7 +294,7 @@ __global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_filter_field(
 template <class G, bool MIX>
 __global__ __launch_bounds__(4 * G::P * G::SPMAX) void k_seg_grad(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int d, int SP, const double *q, typename std::conditional<MIX, float, double>::type *grad,
+    int ntiles, int SP, int d, const double *q, typename std::conditional<MIX, float, double>::type *grad,
     int64_t NT)
 {
     SEG_PROLOGUE(ntiles)
@@ -284,8 +306,8 @@ __global__ __launch_bounds__(4 * G::P * G::SPMAX) void k_seg_grad(
 
 template <class G, bool MIX>
 __global__ __launch_bounds__(5 * G::P * G::SPMAX) void k_seg_flux(
-    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, const InstDev *__restrict__ inst, SegTabs tb,
-    const Tile *__restrict__ tiles, int ntiles, int d, int SP, const double *q,
+    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
+    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *q,
     const typename std::conditional<MIX, float, double>::type *tau, const double *met, double *de, double *drva,
     int64_t NT, int64_t FT, int vis)
 {
@@ -889,10 +911,15 @@ struct cnd_ctx {
     Tile *dtiles[3] = {nullptr, nullptr, nullptr};
     // segmented sweeps: tiles per direction, segments per pencil (rounded to
     // whole waves; 0: the direction uses the tiled sweeps), homogeneous tables
-    std::vector<Tile> hstiles[3];
-    Tile *dstiles[3] = {nullptr, nullptr, nullptr};
-    int seg_sp[3] = {0, 0, 0};
-    double *dsegtab = nullptr;
+    std::vector<Tile> hstiles[kNumP][3];            // tiles of P pencils
+    Tile *dstiles[kNumP][3] = {};
+    double *dseg[kNumL] = {};                       // homogeneous solutions per L
+    // sweep shape per kind (CND_KIND_*) and direction: index into kShapes
+    // or kTiled, with the segments per pencil of the launch (seg_sp)
+    int shape[CND_NKIND][3] = {};
+    int seg_sp[CND_NKIND][3] = {};
+    bool tuned = false, tune_on = true, tiled_only = false;
+    double tune_ms[CND_NKIND][3] = {};
     std::vector<Tile> hpt;
     Tile *dpt = nullptr;
     double *qa = nullptr, *qo = nullptr, *de = nullptr, *grad = nullptr, *tau = nullptr, *met = nullptr;
@@ -902,7 +929,7 @@ struct cnd_ctx {
     double *drva[3] = {}, *cmf[3] = {}, *surf[3] = {};
     // mixed precision: FP32 gradients and stresses; FP32 LU / homogeneous
     // tables and the FP64 banded LHS for the refinement
-    float *gradf = nullptr, *tauf = nullptr, *dlu32 = nullptr, *dseg32 = nullptr;
+    float *gradf = nullptr, *tauf = nullptr, *dlu32 = nullptr, *dseg32[kNumL] = {};
     double *dlhs = nullptr;
     JuncDev jc, je;
     std::vector<char> grid_set;
@@ -1013,34 +1040,67 @@ struct Section {
 };
 
 inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
-inline int nstiles(cnd_ctx *c, int d) { return int(c->hstiles[d].size()); }
-
-// Launch a segmented sweep K<G, MIX> (NV variables) along d with the shape of d.
-#define LAUNCH_SEG(K, MIX, NV, c, d, y, ...)                                                                \
+// Launch the segmented sweep K<G, MIX> (NV variables) for kind along d with
+// the shape chosen for them; the arguments after the common ones follow.
+#define SEG_CASE(i, K, MIX, NV, c, kind, d, y, ...)                                                        \
+    case i: {                                                                                              \
+        using G = Sh##i;                                                                                   \
+        const int sp_ = (c)->seg_sp[kind][d];                                                              \
+        const int pi_ = p_index(G::P);                                                                     \
+        const int nt_ = int((c)->hstiles[pi_][d].size());                                                  \
+        hipLaunchKernelGGL((K<G, MIX>), dim3(unsigned(xcd_width(nt_)), unsigned(y)),                       \
+                           dim3(unsigned((NV) * G::P * sp_)),                                              \
+                           (SegShape<NV, 0, G::L, G::P, G::C, G::XI>::lds_bytes(sp_)), (c)->st, (c)->dblk, \
+                           (c)->dtab, seg_tabs(c, G::L), (c)->dstiles[pi_][d], nt_, sp_, d, __VA_ARGS__);   \
+    } break;
+#define LAUNCH_SEG(K, MIX, NV, c, kind, d, y, ...)                                                          \
     do {                                                                                                   \
-        const int sp_ = (c)->seg_sp[d];                                                                    \
-        const dim3 g_(unsigned(xcd_width(nstiles(c, d))), unsigned(y));                                   \
-        if ((d) == 0)                                                                                      \
-            hipLaunchKernelGGL((K<SegXi, MIX>), g_, dim3((NV) * SegXi::P * sp_),                           \
-                               (SegShape<NV, 0, SegXi::L, SegXi::P, SegXi::C, true>::lds_bytes(sp_)),      \
-                               (c)->st, __VA_ARGS__);                                                      \
-        else                                                                                               \
-            hipLaunchKernelGGL((K<SegEta, MIX>), g_, dim3((NV) * SegEta::P * sp_),                         \
-                               (SegShape<NV, 0, SegEta::L, SegEta::P, SegEta::C, false>::lds_bytes(sp_)),  \
-                               (c)->st, __VA_ARGS__);                                                      \
+        switch ((c)->shape[kind][d]) {                                                                     \
+            SEG_CASE(0, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            SEG_CASE(1, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            SEG_CASE(2, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            SEG_CASE(3, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            SEG_CASE(4, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            SEG_CASE(5, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            SEG_CASE(6, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            default: break;                                                                                \
+        }                                                                                                  \
     } while (0)
 
-inline SegTabs seg_tabs(cnd_ctx *c) { return SegTabs{c->dlu, c->dsegtab, c->dlhs, c->dlu32, c->dseg32}; }
+inline SegTabs seg_tabs(cnd_ctx *c, int L)
+{
+    return SegTabs{c->dlu, c->dseg[l_index(L)], c->dlhs, c->dlu32, c->dseg32[l_index(L)]};
+}
+inline bool seg_on(cnd_ctx *c, int kind, int d) { return c->shape[kind][d] != kTiled; }
 
-// mix: FP32 solve with FP64 refinement (segmented sweeps only)
+// Segments per pencil a launch of shape sh along d needs (the largest over
+// blocks, rounded up to whole waves per variable where that fits), or 0 when
+// some block's pencils do not fit the shape.
+int shape_sp(cnd_ctx *c, int sh, int d)
+{
+    const ShapeInfo &G = kShapes[sh];
+    if (G.XI != (d == 0)) return 0;
+    int sp = 1;
+    for (const BlockSpec &B : c->spec) {
+        const int S = seg_count(B.n[d], G.L);
+        if (S == 0) return 0;
+        sp = std::max(sp, S);
+    }
+    if (sp > G.SPMAX) return 0;
+    const int per_wave = 64 / G.P;
+    const int up = (sp + per_wave - 1) / per_wave * per_wave;
+    return up <= G.SPMAX ? up : sp;
+}
+
+// Filter (kind CND_KIND_FILTER) and plain derivative (CND_KIND_FIELD) sweeps
+// of nvar fields; mix: FP32 solves with FP64 refinement (segmented only).
 int run_filter_field(cnd_ctx *c, int d, const double *in, double *out, int nvar, bool mix = false)
 {
+    const int K = CND_KIND_FILTER;
     if (mix)
-        LAUNCH_SEG(k_seg_filter_field, true, 1, c, d, nvar, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
-                   nstiles(c, d), d, c->seg_sp[d], in, out, c->NT);
-    else if (c->seg_sp[d] > 0)
-        LAUNCH_SEG(k_seg_filter_field, false, 1, c, d, nvar, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
-                   nstiles(c, d), d, c->seg_sp[d], in, out, c->NT);
+        LAUNCH_SEG(k_seg_filter_field, true, 1, c, K, d, nvar, in, out, c->NT);
+    else if (seg_on(c, K, d))
+        LAUNCH_SEG(k_seg_filter_field, false, 1, c, K, d, nvar, in, out, c->NT);
     else
         LAUNCH_SWEEP(k_filter_field, 1, c, d, nvar, kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d,
                      c->nmax[d], c->ys, in, out, c->NT);
@@ -1050,15 +1110,45 @@ int run_filter_field(cnd_ctx *c, int d, const double *in, double *out, int nvar,
 int run_fd_field(cnd_ctx *c, int d, int n45, const double *in, int64_t ivs, double *out, int64_t ovs, int nvar,
                  bool mix = false)
 {
+    const int K = CND_KIND_FIELD;
     if (mix)
-        LAUNCH_SEG(k_seg_fd_field, true, 1, c, d, nvar, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d], nstiles(c, d),
-                   d, n45, c->seg_sp[d], in, ivs, out, ovs);
-    else if (c->seg_sp[d] > 0)
-        LAUNCH_SEG(k_seg_fd_field, false, 1, c, d, nvar, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
-                   nstiles(c, d), d, n45, c->seg_sp[d], in, ivs, out, ovs);
+        LAUNCH_SEG(k_seg_fd_field, true, 1, c, K, d, nvar, n45, in, ivs, out, ovs);
+    else if (seg_on(c, K, d))
+        LAUNCH_SEG(k_seg_fd_field, false, 1, c, K, d, nvar, n45, in, ivs, out, ovs);
     else
         LAUNCH_SWEEP(k_fd_field, 1, c, d, nvar, kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d, n45,
                      c->nmax[d], c->ys, in, ivs, out, ovs);
+    return check_launch(c);
+}
+
+// Gradient sweep along d of the state cur (kind CND_KIND_GRAD).
+int run_grad(cnd_ctx *c, int d, const double *cur)
+{
+    const int K = CND_KIND_GRAD;
+    if (c->precision == CND_MIXED)
+        LAUNCH_SEG(k_seg_grad, true, 4, c, K, d, 1, cur, c->gradf, c->NT);
+    else if (seg_on(c, K, d))
+        LAUNCH_SEG(k_seg_grad, false, 4, c, K, d, 1, cur, c->grad, c->NT);
+    else
+        LAUNCH_SWEEP(k_fd_grad, 4, c, d, 1, 4 * kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d,
+                     c->nmax[d], c->ys, cur, c->grad, c->NT);
+    return check_launch(c);
+}
+
+// Flux sweep along d of the state cur into de (kind CND_KIND_FLUX).
+int run_flux(cnd_ctx *c, int d, const double *cur)
+{
+    const int K = CND_KIND_FLUX;
+    if (c->precision == CND_MIXED)
+        LAUNCH_SEG(k_seg_flux, true, 5, c, K, d, 1, c->dinst, cur, c->tauf, c->met, c->de, c->drva[d], c->NT,
+                   c->FT[d], int(c->any_visc));
+    else if (seg_on(c, K, d))
+        LAUNCH_SEG(k_seg_flux, false, 5, c, K, d, 1, c->dinst, cur, c->tau, c->met, c->de, c->drva[d], c->NT,
+                   c->FT[d], int(c->any_visc));
+    else
+        LAUNCH_SWEEP(k_fd_flux, 5, c, d, 1, 5 * kTile, c->dblk, c->dtab, c->dinst, c->dlu, c->dtiles[d],
+                     ntiles(c, d), d, c->nmax[d], c->ys, cur, c->tau, c->met, c->de, c->drva[d], c->NT, c->FT[d],
+                     int(c->any_visc));
     return check_launch(c);
 }
 
@@ -1092,18 +1182,8 @@ int enqueue_stage(cnd_ctx *c, int nk)
     int r;
     if (c->any_visc) {
         Section sec(c, CND_SEC_GRAD, 3);
-        for (int d = 0; d < 3; ++d) {
-            if (c->precision == CND_MIXED) {
-                LAUNCH_SEG(k_seg_grad, true, 4, c, d, 1, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
-                           nstiles(c, d), d, c->seg_sp[d], cur, c->gradf, c->NT);
-            } else if (c->seg_sp[d] > 0) {
-                LAUNCH_SEG(k_seg_grad, false, 4, c, d, 1, c->dblk, c->dtab, seg_tabs(c), c->dstiles[d],
-                           nstiles(c, d), d, c->seg_sp[d], cur, c->grad, c->NT);
-            } else {
-                LAUNCH_SWEEP(k_fd_grad, 4, c, d, 1, 4 * kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d),
-                             d, c->nmax[d], c->ys, cur, c->grad, c->NT);
-            }
-        }
+        for (int d = 0; d < 3; ++d)
+            if ((r = run_grad(c, d, cur))) return r;
     }
     if (c->any_visc) {
         Section sec(c, CND_SEC_STRESS, 1);
@@ -1116,21 +1196,8 @@ int enqueue_stage(cnd_ctx *c, int nk)
     }
     {
     Section sec(c, CND_SEC_FLUX, 3);
-    for (int d = 0; d < 3; ++d) {
-        if (c->precision == CND_MIXED) {
-            LAUNCH_SEG(k_seg_flux, true, 5, c, d, 1, c->dblk, c->dtab, c->dinst, seg_tabs(c), c->dstiles[d],
-                       nstiles(c, d), d, c->seg_sp[d], cur, c->tauf, c->met, c->de, c->drva[d], c->NT, c->FT[d],
-                       int(c->any_visc));
-        } else if (c->seg_sp[d] > 0) {
-            LAUNCH_SEG(k_seg_flux, false, 5, c, d, 1, c->dblk, c->dtab, c->dinst, seg_tabs(c), c->dstiles[d],
-                       nstiles(c, d), d, c->seg_sp[d], cur, c->tau, c->met, c->de, c->drva[d], c->NT, c->FT[d],
-                       int(c->any_visc));
-        } else {
-            LAUNCH_SWEEP(k_fd_flux, 5, c, d, 1, 5 * kTile, c->dblk, c->dtab, c->dinst, c->dlu, c->dtiles[d],
-                         ntiles(c, d), d, c->nmax[d], c->ys, cur, c->tau, c->met, c->de, c->drva[d], c->NT, c->FT[d],
-                         int(c->any_visc));
-        }
-    }
+    for (int d = 0; d < 3; ++d)
+        if ((r = run_flux(c, d, cur))) return r;
     }
     if (c->any_gcbc) {
         Section sec(c, CND_SEC_GCBC, 9);
@@ -1191,6 +1258,17 @@ int enqueue_step(cnd_ctx *c)
 }  // namespace
 
 // ==== C ABI =========================================================================
+int cnd::ctx_block_view(cnd_ctx *c, int b, CtxView &v)
+{
+    if (!c || b < 0 || b >= c->nblk) return CND_EINVAL;
+    v.q = c->state_in_qo ? c->qo : c->qa;
+    v.NT = c->NT;
+    v.off = c->off[b];
+    for (int d = 0; d < 3; ++d) v.n[d] = c->spec[b].n[d];
+    v.st = c->st;
+    return CND_OK;
+}
+
 extern "C" {
 
 const char *cnd_last_error(const cnd_ctx *c) { return c ? c->err.c_str() : g_noctx_err.c_str(); }
@@ -1234,7 +1312,7 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     for (int i = 0; i < ninst; ++i) build_tables(params[i].fltk, params[i].fltrbc, c->htab[i]);
 
     // layout
-    std::vector<double> lu, segtab, lhs;
+    std::vector<double> lu, segtab[kNumL], lhs;
     c->hblk.resize(nblocks);
     int64_t off = 0, foff[3] = {0, 0, 0};
     for (int b = 0; b < nblocks; ++b) {
@@ -1266,8 +1344,8 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
             const int np = npencils(B, d);
             for (int q0 = 0; q0 < np; q0 += kLine) c->htiles[d].push_back(Tile{b, q0});
             foff[d] += np;
-            const int PL = d == 0 ? SegXi::P : SegEta::P, SL = d == 0 ? SegXi::L : SegEta::L;
-            for (int q0 = 0; q0 < np; q0 += PL) c->hstiles[d].push_back(Tile{b, q0});
+            for (int pi = 0; pi < kNumP; ++pi)
+                for (int q0 = 0; q0 < np; q0 += kPs[pi]) c->hstiles[pi][d].push_back(Tile{b, q0});
             for (int sch = 0; sch < 2; ++sch) {
                 B.lu[sch][d] = int(lu.size() / 5);
                 const size_t at = lu.size();
@@ -1275,11 +1353,15 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
                 lu_factors(c->htab[s.inst], sch, s.n[d], B.qd[d][0] != 0, B.qd[d][1] != 0, lu.data() + at);
                 lhs.resize(lu.size());
                 lhs_rows(c->htab[s.inst], sch, s.n[d], B.qd[d][0] != 0, B.qd[d][1] != 0, lhs.data() + at);
-                segtab.resize(4 * (at / 5 + size_t(s.n[d])), 0.0);
-                const int nd = s.n[d], S = seg_count(nd, SL);
-                if (S > 0)
-                    seg_tables(lu.data() + at, S, [&](int k, int &a0, int &l0) { seg_bounds(nd, SL, S, k, a0, l0); },
-                               segtab.data() + 4 * (at / 5));
+                const int nd = s.n[d];
+                for (int li = 0; li < kNumL; ++li) {
+                    const int SL = kLs[li], S = seg_count(nd, SL);
+                    segtab[li].resize(4 * (at / 5 + size_t(nd)), 0.0);
+                    if (S > 0)
+                        seg_tables(lu.data() + at, S,
+                                   [&](int k, int &a0, int &l0) { seg_bounds(nd, SL, S, k, a0, l0); },
+                                   segtab[li].data() + 4 * (at / 5));
+                }
             }
         }
         for (int64_t p0 = 0; p0 < npt; p0 += kPts) c->hpt.push_back(Tile{b, int(p0)});
@@ -1289,26 +1371,32 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     for (int d = 0; d < 3; ++d) c->FT[d] = foff[d];
     for (int b = 0; b < nblocks; ++b)
         for (int d = 0; d < 3; ++d) c->nmax[d] = std::max(c->nmax[d], c->spec[b].n[d]);
-    // segmented sweeps where every block's pencils of direction d fit the
-    // shape (CANARD_SWEEP=tiled keeps the tiled sweeps everywhere)
+    // sweep shapes: the default segmented shape of each direction where every
+    // block's pencils fit it (else another compiled shape, else the tiled
+    // sweep); CANARD_SWEEP=tiled keeps the tiled sweeps everywhere and
+    // CANARD_TUNE=0 keeps these defaults (no autotuning)
     {
         const char *sw = std::getenv("CANARD_SWEEP");
-        const bool tiled_only = sw && std::string(sw) == "tiled";
+        c->tiled_only = sw && std::string(sw) == "tiled";
+        const char *tu = std::getenv("CANARD_TUNE");
+        c->tune_on = !c->tiled_only && !(tu && std::string(tu) == "0");
         for (int d = 0; d < 3; ++d) {
-            const int SL = d == 0 ? SegXi::L : SegEta::L, PL = d == 0 ? SegXi::P : SegEta::P;
-            const int SPM = d == 0 ? SegXi::SPMAX : SegEta::SPMAX, per_wave = 64 / PL;
-            int sp = tiled_only ? 0 : 1;
-            for (int b = 0; b < nblocks && sp > 0; ++b) {
-                const int S = seg_count(c->spec[b].n[d], SL);
-                sp = S == 0 ? 0 : std::max(sp, S);
+            const int dflt = d == 0 ? 0 : 2;
+            int pick = kTiled;
+            if (!c->tiled_only) {
+                if (shape_sp(c, dflt, d) > 0) pick = dflt;
+                for (int sh = 0; pick == kTiled && sh < kNumShapes; ++sh)
+                    if (shape_sp(c, sh, d) > 0) pick = sh;
             }
-            if (sp > 0) sp = (sp + per_wave - 1) / per_wave * per_wave;
-            c->seg_sp[d] = sp <= SPM ? sp : 0;
-            if (precision == CND_MIXED && c->seg_sp[d] == 0) {
+            for (int k = 0; k < CND_NKIND; ++k) {
+                c->shape[k][d] = pick;
+                c->seg_sp[k][d] = pick == kTiled ? 0 : shape_sp(c, pick, d);
+            }
+            if (precision == CND_MIXED && pick == kTiled) {
                 delete c;
                 return fail(nullptr, CND_EINVAL,
-                            "mixed precision needs the segmented sweeps in every direction (pencils of 13..272 (xi) "
-                            "or 13..264 (eta, zeta) points, and CANARD_SWEEP not set to tiled)");
+                            "mixed precision needs the segmented sweeps in every direction (pencils of 13..528 "
+                            "points, and CANARD_SWEEP not set to tiled)");
             }
         }
     }
@@ -1330,14 +1418,18 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
                 ok(dalloc(c, &c->dlu, lu.size())) && ok(dalloc(c, &c->qa, 5 * NT)) && ok(dalloc(c, &c->qo, 5 * NT)) &&
                 ok(dalloc(c, &c->de, 5 * NT)) && ok(dalloc(c, &c->grad, 12 * NT)) && ok(dalloc(c, &c->tau, 9 * NT)) &&
                 ok(dalloc(c, &c->met, 10 * NT)) && ok(dalloc(c, &c->ys, c->ys_len)) && ok(dalloc(c, &c->xyz, 3 * NT)) &&
-                ok(dalloc(c, &c->sp_par, 5 * ninst)) && ok(dalloc(c, &c->dsegtab, segtab.size()));
+                ok(dalloc(c, &c->sp_par, 5 * ninst)) && ok(dalloc(c, &c->dseg[0], segtab[0].size())) &&
+                ok(dalloc(c, &c->dseg[1], segtab[1].size()));
     if (good && c->any_sponge) good = ok(dalloc(c, &c->sponge, 2 * NT));
     if (good && precision == CND_MIXED)
         good = ok(dalloc(c, &c->gradf, 12 * NT)) && ok(dalloc(c, &c->tauf, 9 * NT)) &&
-               ok(dalloc(c, &c->dlu32, lu.size())) && ok(dalloc(c, &c->dseg32, segtab.size())) &&
+               ok(dalloc(c, &c->dlu32, lu.size())) && ok(dalloc(c, &c->dseg32[0], segtab[0].size())) &&
+               ok(dalloc(c, &c->dseg32[1], segtab[1].size())) &&
                ok(dalloc(c, &c->dlhs, lhs.size()));
     for (int d = 0; good && d < 3; ++d)
-        good = ok(dalloc(c, &c->dtiles[d], c->htiles[d].size())) && ok(dalloc(c, &c->dstiles[d], c->hstiles[d].size())) && ok(dalloc(c, &c->drva[d], 10 * c->FT[d])) &&
+        good = ok(dalloc(c, &c->dtiles[d], c->htiles[d].size())) && ok(dalloc(c, &c->dstiles[0][d], c->hstiles[0][d].size())) &&
+               ok(dalloc(c, &c->dstiles[1][d], c->hstiles[1][d].size())) &&
+               ok(dalloc(c, &c->dstiles[2][d], c->hstiles[2][d].size())) && ok(dalloc(c, &c->drva[d], 10 * c->FT[d])) &&
                ok(dalloc(c, &c->cmf[d], 6 * c->FT[d])) && ok(dalloc(c, &c->surf[d], 10 * c->FT[d]));
     if (good) good = ok(dalloc(c, &c->dpt, c->hpt.size())) && ok(dalloc(c, &c->tmax, 2 * c->hpt.size()));
     if (!good) { std::string m = c->err; cnd_destroy(c); return fail(nullptr, r ? r : CND_ENOMEM, m); }
@@ -1368,12 +1460,17 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     up(c->dlu, lu.data(), sizeof(double) * lu.size());
     up(c->sp_par, sp.data(), sizeof(double) * sp.size());
     for (int d = 0; d < 3; ++d) up(c->dtiles[d], c->htiles[d].data(), sizeof(Tile) * c->htiles[d].size());
-    for (int d = 0; d < 3; ++d) up(c->dstiles[d], c->hstiles[d].data(), sizeof(Tile) * c->hstiles[d].size());
-    up(c->dsegtab, segtab.data(), sizeof(double) * segtab.size());
+    for (int pi = 0; pi < kNumP; ++pi)
+        for (int d = 0; d < 3; ++d)
+            up(c->dstiles[pi][d], c->hstiles[pi][d].data(), sizeof(Tile) * c->hstiles[pi][d].size());
+    for (int li = 0; li < kNumL; ++li) up(c->dseg[li], segtab[li].data(), sizeof(double) * segtab[li].size());
     if (precision == CND_MIXED) {
-        const std::vector<float> lu32(lu.begin(), lu.end()), seg32(segtab.begin(), segtab.end());
+        const std::vector<float> lu32(lu.begin(), lu.end());
         up(c->dlu32, lu32.data(), sizeof(float) * lu32.size());
-        up(c->dseg32, seg32.data(), sizeof(float) * seg32.size());
+        for (int li = 0; li < kNumL; ++li) {
+            const std::vector<float> seg32(segtab[li].begin(), segtab[li].end());
+            up(c->dseg32[li], seg32.data(), sizeof(float) * seg32.size());
+        }
         up(c->dlhs, lhs.data(), sizeof(double) * lhs.size());
     }
     up(c->dpt, c->hpt.data(), sizeof(Tile) * c->hpt.size());
@@ -1381,6 +1478,7 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     if (!r) {
         hipError_t e = hipMemset(c->de, 0, sizeof(double) * 5 * NT);
         if (e == hipSuccess) e = hipMemset(c->tau, 0, sizeof(double) * 9 * NT);
+        if (e == hipSuccess && c->tauf) e = hipMemset(c->tauf, 0, sizeof(float) * 9 * NT);
         if (e != hipSuccess) r = fail(c, CND_EDEVICE, "memset");
     }
     // junction plans (search_point / search_line)
@@ -1416,12 +1514,13 @@ void cnd_destroy(cnd_ctx *c)
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->tau, c->met, c->ys,
-                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->tauf, c->dlu32, c->dseg32, c->dlhs, c->jc.tgt, c->jc.src, c->jc.first, c->jc.scale,
-                    c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp, c->dsegtab};
+                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->tauf, c->dlu32, c->dseg32[0], c->dseg32[1], c->dlhs, c->jc.tgt, c->jc.src, c->jc.first, c->jc.scale,
+                    c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp, c->dseg[0], c->dseg[1]};
     for (void *p : ptrs) if (p) hipFree(p);
     for (int d = 0; d < 3; ++d) {
         if (c->dtiles[d]) hipFree(c->dtiles[d]);
-        if (c->dstiles[d]) hipFree(c->dstiles[d]);
+        for (int pi = 0; pi < kNumP; ++pi)
+            if (c->dstiles[pi][d]) hipFree(c->dstiles[pi][d]);
         if (c->drva[d]) hipFree(c->drva[d]);
         if (c->cmf[d]) hipFree(c->cmf[d]);
         if (c->surf[d]) hipFree(c->surf[d]);
@@ -1544,11 +1643,119 @@ static int ready(cnd_ctx *c)
     return CND_OK;
 }
 
+// Autotuner: for each sweep kind of the step and each direction, times every
+// compiled segmented shape that fits the blocks (and, in FP64, the tiled
+// sweep) on the context's own arrays and keeps the fastest.  All candidates
+// compute the same values up to rounding.  Runs once, before the first step.
+static int tune(cnd_ctx *c)
+{
+    if (c->tuned) return CND_OK;
+    c->tuned = true;
+    if (!c->tune_on) return CND_OK;
+    const bool mix = c->precision == CND_MIXED;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPX(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(c, CND_EDEVICE, "tune: event"); }
+    int r = CND_OK;
+    auto run = [&](int kind, int d) {
+        if (kind == CND_KIND_FILTER) return run_filter_field(c, d, c->qa, c->tau, 5, mix);
+        if (kind == CND_KIND_GRAD) return run_grad(c, d, c->qa);
+        return run_flux(c, d, c->qa);
+    };
+    for (int kind : {int(CND_KIND_FILTER), int(CND_KIND_GRAD), int(CND_KIND_FLUX)}) {
+        if (kind == CND_KIND_GRAD && !c->any_visc) continue;
+        for (int d = 0; d < 3 && !r; ++d) {
+            std::vector<int> cands;
+            if (!mix) cands.push_back(kTiled);
+            for (int sh = 0; sh < kNumShapes; ++sh)
+                if (shape_sp(c, sh, d) > 0) cands.push_back(sh);
+            if (cands.size() < 2) continue;
+            int best = c->shape[kind][d];
+            double best_ms = 1e300;
+            for (int cand : cands) {
+                c->shape[kind][d] = cand;
+                c->seg_sp[kind][d] = cand == kTiled ? 0 : shape_sp(c, cand, d);
+                constexpr int reps = 3;
+                if ((r = run(kind, d))) break;
+                (void)hipEventRecord(e0, c->st);
+                for (int i = 0; i < reps && !r; ++i) r = run(kind, d);
+                (void)hipEventRecord(e1, c->st);
+                if (r) break;
+                float ms = 0.f;
+                if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    r = fail(c, CND_EDEVICE, "tune: timing");
+                    break;
+                }
+                if (ms / reps < best_ms) { best_ms = ms / reps; best = cand; }
+            }
+            c->shape[kind][d] = best;
+            c->seg_sp[kind][d] = best == kTiled ? 0 : shape_sp(c, best, d);
+            c->tune_ms[kind][d] = best_ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return r;
+}
+
+int cnd_tune(cnd_ctx *c)
+{
+    int r;
+    if ((r = ready(c))) return r;
+    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_tune: a step is half done");
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    c->tuned = false;
+    return tune(c);
+}
+
+int cnd_shape_info(int shape, int *seg_len, int *pencils, int *run, int *xi)
+{
+    if (shape < 0 || shape >= kNumShapes) return CND_EINVAL;
+    const ShapeInfo &G = kShapes[shape];
+    if (seg_len) *seg_len = G.L;
+    if (pencils) *pencils = G.P;
+    if (run) *run = G.C;
+    if (xi) *xi = G.XI ? 1 : 0;
+    return CND_OK;
+}
+
+int cnd_set_shape(cnd_ctx *c, int kind, int dir, int shape)
+{
+    if (!c || kind < 0 || kind >= CND_NKIND || dir < 0 || dir > 2 || shape < -1 || shape >= kNumShapes)
+        return fail(c, CND_EINVAL, "cnd_set_shape: bad argument");
+    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_set_shape: a step is half done");
+    if (shape == kTiled && c->precision == CND_MIXED)
+        return fail(c, CND_EINVAL, "cnd_set_shape: mixed precision runs on the segmented sweeps only");
+    const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir);
+    if (shape != kTiled && sp == 0)
+        return fail(c, CND_EINVAL, "cnd_set_shape: the shape does not fit the pencils of this direction");
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    c->shape[kind][dir] = shape;
+    c->seg_sp[kind][dir] = sp;
+    c->tuned = true;   // an explicit choice is kept
+    c->tune_ms[kind][dir] = 0.0;
+    return CND_OK;
+}
+
+int cnd_tuning(const cnd_ctx *c, int kind, int dir, int *shape, int *seg_len, int *pencils, int *segments,
+               double *ms)
+{
+    if (!c || kind < 0 || kind >= CND_NKIND || dir < 0 || dir > 2) return CND_EINVAL;
+    const int sh = c->shape[kind][dir];
+    if (shape) *shape = sh;
+    if (seg_len) *seg_len = sh == kTiled ? 0 : kShapes[sh].L;
+    if (pencils) *pencils = sh == kTiled ? kTile : kShapes[sh].P;
+    if (segments) *segments = sh == kTiled ? 1 : c->seg_sp[kind][dir];
+    if (ms) *ms = c->tune_ms[kind][dir];
+    return CND_OK;
+}
+
 int cnd_filter(cnd_ctx *c)
 {
     int r;
     if ((r = ready(c))) return r;
     if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_filter: the filter phase already ran for this step");
+    if ((r = tune(c))) return r;
     if ((r = enqueue_filter(c))) return r;
     c->state_in_qo = true;
     return CND_OK;
@@ -1560,6 +1767,7 @@ int cnd_rk_stage(cnd_ctx *c, int nk)
     if ((r = ready(c))) return r;
     if (nk < 1 || nk > 4) return fail(c, CND_EINVAL, "cnd_rk_stage: nk must be 1..4");
     if ((nk == 1) != c->state_in_qo) return fail(c, CND_ESTATE, "cnd_rk_stage: stage 1 must follow cnd_filter");
+    if ((r = tune(c))) return r;
     if ((r = enqueue_stage(c, nk))) return r;
     c->state_in_qo = false;
     return CND_OK;
@@ -1585,6 +1793,7 @@ int cnd_step(cnd_ctx *c, int nsteps)
     if ((r = ready(c))) return r;
     if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_step: a step is half done (cnd_filter without stages)");
     if (nsteps < 0) return fail(c, CND_EINVAL, "cnd_step: negative step count");
+    if ((r = tune(c))) return r;
     if (!c->use_graph) {
         for (int s = 0; s < nsteps; ++s)
             if ((r = enqueue_step(c))) return r;

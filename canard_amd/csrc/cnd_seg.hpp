@@ -35,25 +35,25 @@ constexpr int kSegEnd = 12;
 constexpr int kSegEndRows = kSegEnd + 1;
 
 // Segments of an n-point pencil for lanes of at most L points: 0 when no
-// partition fits.  With S > 1 the last segment holds exactly the last L
-// points and the first n - L points are shared evenly by the others; the
-// first segment must hold the 12 points its closure rows use.
+// partition fits (a segment holding a pencil end must contain the 12 points
+// its closure rows use).  Segments are as even as possible.
 __host__ __device__ inline int seg_count(int n, int L)
 {
-    if (n <= L) return 1;
     const int S = (n + L - 1) / L;
-    if ((n - L) / (S - 1) < kSegEnd) return 0;
+    if (S > 1 && n / S < kSegEnd) return 0;
     return S;
 }
 
-__host__ __device__ inline void seg_bounds(int n, int L, int S, int s, int &a, int &len)
+__host__ __device__ inline void seg_bounds(int n, int, int S, int s, int &a, int &len)
 {
-    if (S == 1) { a = 0; len = n; return; }
-    if (s == S - 1) { a = n - L; len = L; return; }
-    const int m = n - L, q = m / (S - 1), r = m % (S - 1);
+    const int q = n / S, r = n % S;
     a = s * q + (s < r ? s : r);
     len = q + (s < r ? 1 : 0);
 }
+
+// Register index of a segment's first point: the last segment of a split
+// pencil is right-aligned, so the pencil's last point sits at index L-1.
+__host__ __device__ inline int seg_kb(int L, int S, int s, int len) { return (S > 1 && s == S - 1) ? L - len : 0; }
 
 template <int NV, int SCH, int L, int P, int C, bool XI>
 struct SegShape {
@@ -87,7 +87,7 @@ __device__ __forceinline__ void seg_slot(int slot, int SP, int &o, int &s, int &
 
 // One lane's view of its segment.
 struct SegLane {
-    int a, len, s, S, n, L, ok, exi, exs, v, p, P, SP;
+    int a, len, kb, s, S, n, L, ok, exi, exs, v, p, P, SP;
 };
 
 // Solve the line for the lane's segment held in Y (right-hand side in,
@@ -97,13 +97,13 @@ template <class T, int L>
 __device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const T *__restrict__ LU,
                                           const T *__restrict__ ST, double *ex)
 {
-    const int a = ln.a, len = ln.len, s = ln.s;
+    const int a = ln.a - ln.kb, lo = ln.kb, hi = ln.kb + ln.len, s = ln.s;   // row = a + k, lo <= k < hi
     // forward recurrence with zero incoming values
     {
         T ya = T(0), yb = T(0);
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            if (k < len) {
+            if (k >= lo && k < hi) {
                 const T *lr = LU + 5 * (a + k);
                 const T y = (Y[k] - lr[0] * ya) - lr[1] * yb;
                 Y[k] = y;
@@ -131,7 +131,7 @@ __device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const T 
         }
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            if (k < len) {
+            if (k >= lo && k < hi) {
                 const T *ph = ST + 4 * (a + k);
                 Y[k] = Y[k] + (ph[0] * t0 + ph[1] * t1);
             }
@@ -142,7 +142,7 @@ __device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const T 
         T xa = T(0), xb = T(0);
 #pragma unroll
         for (int k = L - 1; k >= 0; --k) {
-            if (k < len) {
+            if (k >= lo && k < hi) {
                 const T *lr = LU + 5 * (a + k);
                 const T x = (lr[2] * Y[k] - lr[3] * xa) - lr[4] * xb;
                 Y[k] = x;
@@ -170,7 +170,7 @@ __device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const T 
         }
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            if (k < len) {
+            if (k >= lo && k < hi) {
                 const T *ps = ST + 4 * (a + k);
                 Y[k] = Y[k] + (ps[2] * h0 + ps[3] * h1);
             }
@@ -215,6 +215,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     const bool lane_ok = pen_ok && s < S;
     int a = 0, len = 0;
     if (s < S) seg_bounds(n, L, S, s, a, len);
+    const int kb = seg_kb(L, S, s, len);   // row a + k - kb is held in Y[k], kb <= k < kb + len
     auto pt = [&](int pp, int i) -> int64_t {
         int64_t pb, st;
         pencil_of(B, d, tile.q0 + pp, pb, st);
@@ -241,8 +242,8 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             if (ss >= S || tile.q0 + pp >= npen) continue;
             int sa, sl;
             seg_bounds(n, L, S, ss, sa, sl);
-            const int row = sa - H + c * C + o;
-            if (row < 0 || row > N || row >= sa + sl + H) continue;
+            const int row = sa - seg_kb(L, S, ss, sl) - H + c * C + o;
+            if (row < 0 || row > N || row < sa - H || row >= sa + sl + H) continue;
             typename Src::Raw raw;
             src.load(pt(pp, row), b, raw);
             double f[NV];
@@ -253,21 +254,21 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         __syncthreads();
 #pragma unroll
         for (int o = 0; o < C; ++o) {
-            const int kr = c * C + o;            // streamed point a - H + kr
+            const int kr = c * C + o;            // streamed point a - kb - H + kr
             if (kr >= Sh::KR) break;
             const double fv = buf[((o * NV + v) * SP + s) * P + p];
-            const int row = a - H + kr;
-            if (lane_ok && row >= 0 && row <= N) {
+            const int row = a - kb - H + kr;
+            if (lane_ok && row >= 0 && row <= N && row >= a - H && row < a + len + H) {
                 if (s == 0 && row <= kSegEnd) ends[eidx(0, row, v, p)] = fv;
                 if (s == S - 1 && N - row <= kSegEnd) ends[eidx(1, N - row, v, p)] = fv;
             }
 #pragma unroll
             for (int m = 0; m < 2 * H; ++m) w[m] = w[m + 1];
             w[2 * H] = fv;
-            const int k = kr - 2 * H;            // row a + k has its stencil complete
+            const int k = kr - 2 * H;            // row a - kb + k has its stencil complete
             if (k < 0 || k >= L) continue;
-            const int j = a + k;
-            if (k < len && j >= jfirst && j <= jlast) {
+            const int j = a - kb + k;
+            if (k >= kb && k < kb + len && j >= jfirst && j <= jlast) {
                 double rr;
                 if constexpr (SCH == 0) {
                     rr = kAa * (w[3] - w[1]) + kAb * (w[4] - w[0]);
@@ -417,7 +418,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
 
     // ---- 4. solve: FP64, or FP32 with one FP64 residual refinement ----------
     double *ex = buf;
-    const SegLane ln{a, len, s, S, n, L, lane_ok, (v * SP + s) * P + p, NV * SP * P, v, p, P, SP};
+    const SegLane ln{a, len, kb, s, S, n, L, lane_ok, (v * SP + s) * P + p, NV * SP * P, v, p, P, SP};
     if constexpr (!MIX) {
         seg_solve<double, L>(Y, ln, LU, ST, ex);
     } else {
@@ -428,15 +429,17 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         // edge values of the neighbouring segments: x(a-2), x(a-1), x(b), x(b+1)
         double l2 = 0.0, l1 = 0.0, r1 = 0.0, r2 = 0.0;
         {
-            double e0 = 0.0, e1 = 0.0;
+            double f0 = 0.0, f1 = 0.0, e0 = 0.0, e1 = 0.0;
 #pragma unroll
             for (int k = 0; k < L; ++k) {
-                if (k == len - 2) e0 = X[k];
-                if (k == len - 1) e1 = X[k];
+                if (k == kb) f0 = X[k];
+                if (k == kb + 1) f1 = X[k];
+                if (k == kb + len - 2) e0 = X[k];
+                if (k == kb + len - 1) e1 = X[k];
             }
             __syncthreads();
-            ex[ln.exi] = double(X[0]);
-            ex[ln.exs + ln.exi] = double(X[1]);
+            ex[ln.exi] = f0;
+            ex[ln.exs + ln.exi] = f1;
             ex[2 * ln.exs + ln.exi] = e0;
             ex[3 * ln.exs + ln.exi] = e1;
             __syncthreads();
@@ -455,12 +458,13 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         const double *AR = tb.lhs + 5 * int64_t(B.lu[SCH][d]);
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            if (k < len) {
-                const double *ar = AR + 5 * (a + k);
-                const double xm2 = k >= 2 ? double(X[k >= 2 ? k - 2 : 0]) : (k == 1 ? l1 : l2);
-                const double xm1 = k >= 1 ? double(X[k >= 1 ? k - 1 : 0]) : l1;
-                const double xp1 = k + 1 < len ? double(X[k + 1 < L ? k + 1 : L - 1]) : r1;
-                const double xp2 = k + 2 < len ? double(X[k + 2 < L ? k + 2 : L - 1]) : (k + 1 < len ? r1 : r2);
+            if (k >= kb && k < kb + len) {
+                const int e = kb + len;
+                const double *ar = AR + 5 * (a - kb + k);
+                const double xm2 = k - 2 >= kb ? double(X[k >= 2 ? k - 2 : 0]) : (k - 1 >= kb ? l1 : l2);
+                const double xm1 = k - 1 >= kb ? double(X[k >= 1 ? k - 1 : 0]) : l1;
+                const double xp1 = k + 1 < e ? double(X[k + 1 < L ? k + 1 : L - 1]) : r1;
+                const double xp2 = k + 2 < e ? double(X[k + 2 < L ? k + 2 : L - 1]) : (k + 1 < e ? r1 : r2);
                 const double ax = (((ar[0] * xm2 + ar[1] * xm1) + double(X[k])) + ar[3] * xp1) + ar[4] * xp2;
                 Y[k] = Y[k] - ax;
             }
@@ -491,10 +495,10 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         if (lane_ok) {
             int64_t pb, st;
             pencil_of(B, d, tile.q0 + p, pb, st);
-            pb += int64_t(a) * st;
+            pb += int64_t(a - kb) * st;
 #pragma unroll
             for (int k = 0; k < L; ++k)
-                if (k < len) sink.put(pb + int64_t(k) * st, b, v, Y[k]);
+                if (k >= kb && k < kb + len) sink.put(pb + int64_t(k) * st, b, v, Y[k]);
         }
     } else {
         // xi: through LDS, stored in runs along the pencil
@@ -504,7 +508,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
 #pragma unroll
             for (int o = 0; o < C; ++o) {
                 const int k = c * C + o;
-                if (k < L && k < len) buf[((o * NV + v) * SP + s) * P + p] = Y[k];
+                if (k < L && k >= kb && k < kb + len) buf[((o * NV + v) * SP + s) * P + p] = Y[k];
             }
             __syncthreads();
             for (int slot = t; slot < nslots; slot += nthr) {
@@ -513,12 +517,12 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
                 if (ss >= S || tile.q0 + pp >= npen) continue;
                 int sa, sl;
                 seg_bounds(n, L, S, ss, sa, sl);
-                const int k = c * C + o;
-                if (k >= sl) continue;
+                const int skb = seg_kb(L, S, ss, sl), k = c * C + o;
+                if (k < skb || k >= skb + sl) continue;
                 double x[NV];
 #pragma unroll
                 for (int m = 0; m < NV; ++m) x[m] = buf[((o * NV + m) * SP + ss) * P + pp];
-                sink.store(pt(pp, sa + k), b, x);
+                sink.store(pt(pp, sa - skb + k), b, x);
             }
         }
     }

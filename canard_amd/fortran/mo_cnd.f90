@@ -12,6 +12,7 @@ module mo_cnd
 
    integer(c_int), parameter :: CND_OK = 0, CND_FP64 = 0, CND_MIXED = 1
    integer(c_int), parameter :: CND_NSEC = 8
+   integer(c_int), parameter :: CND_KIND_FILTER = 0, CND_KIND_GRAD = 1, CND_KIND_FLUX = 2, CND_KIND_FIELD = 3
 
    type, bind(C) :: cnd_params
       real(c_double)   :: fltk, fltrbc
@@ -102,6 +103,23 @@ module mo_cnd
          integer(c_int), value :: section
          real(c_double), intent(out) :: ms
          integer(c_long_long), intent(out) :: launches
+      end function
+      integer(c_int) function cnd_tune(ctx) bind(C, name="cnd_tune")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function cnd_set_shape(ctx, kind, dir, shape) bind(C, name="cnd_set_shape")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: kind, dir, shape
+      end function
+      integer(c_int) function cnd_tuning(ctx, kind, dir, shape, seg_len, pencils, segments, ms) &
+            bind(C, name="cnd_tuning")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: kind, dir
+         integer(c_int), intent(out) :: shape, seg_len, pencils, segments
+         real(c_double), intent(out) :: ms
       end function
    end interface
 

@@ -27,6 +27,17 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def shapes():
+    """Compiled sweep shapes: list of dict(seg_len, pencils, run, xi)."""
+    L = _lib.load()
+    out = []
+    while True:
+        v = [C.c_int() for _ in range(4)]
+        if L.cnd_shape_info(len(out), *[C.byref(x) for x in v]) != 0:
+            return out
+        out.append(dict(seg_len=v[0].value, pencils=v[1].value, run=v[2].value, xi=bool(v[3].value)))
+
+
 class Case:
     """blocks: list of dicts (lxi, let, lze, nbc, mcd[, inst]); params: one
     dict per instance (missing keys take DEFAULT_PARAMS)."""
@@ -144,6 +155,30 @@ class Case:
             ms, n = C.c_double(), C.c_longlong()
             self._check(self.L.cnd_profile_read(self.h, i, C.byref(ms), C.byref(n)), "cnd_profile_read")
             out[name] = (ms.value, n.value)
+        return out
+
+    def tune(self):
+        """Re-run the sweep-shape autotuner now (it runs once by itself before
+        the first step unless CANARD_TUNE=0)."""
+        self._check(self.L.cnd_tune(self.h), "cnd_tune")
+
+    def set_shape(self, kind, d, shape):
+        """Pin the sweep shape of kind ('filter', 'grad', 'flux', 'field') along d
+        (-1: tiled sweep); turns automatic tuning off."""
+        self._check(self.L.cnd_set_shape(self.h, _lib.KINDS.index(kind), d, shape), "cnd_set_shape")
+
+    def tuning(self):
+        """{kind: [per direction dict(shape, seg_len, pencils, segments, ms)]}."""
+        out = {}
+        for k, name in enumerate(_lib.KINDS):
+            rows = []
+            for d in range(3):
+                v = [C.c_int() for _ in range(4)]
+                ms = C.c_double()
+                self._check(self.L.cnd_tuning(self.h, k, d, *[C.byref(x) for x in v], C.byref(ms)), "cnd_tuning")
+                rows.append(dict(shape=v[0].value, seg_len=v[1].value, pencils=v[2].value, segments=v[3].value,
+                                 ms=ms.value))
+            out[name] = rows
         return out
 
     def stream(self):

@@ -141,6 +141,73 @@ enum { CND_SEC_FILTER = 0, CND_SEC_TSTEP, CND_SEC_GRAD, CND_SEC_STRESS, CND_SEC_
 int cnd_profile(cnd_ctx *ctx, int enable);
 int cnd_profile_read(cnd_ctx *ctx, int section, double *ms, long long *launches);
 
+/* Sweep-shape autotuner (no counterpart in the reference, which has no
+ * device kernels).  Kinds: CND_KIND_FILTER (filter phase), CND_KIND_GRAD
+ * (gradient sweeps), CND_KIND_FLUX (flux sweeps), CND_KIND_FIELD (single
+ * field sweeps: metrics, cnd_deriv_sweep).  Before the first step (or on
+ * cnd_tune) every compiled shape that fits the blocks -- segments of L
+ * points (lanes per pencil), pencils per workgroup, staging run length --
+ * and, in FP64, the tiled one-lane-per-pencil sweep are timed on the
+ * context's arrays per kind and direction, and the fastest is kept; all of
+ * them give the same values up to rounding.  CANARD_TUNE=0 in the
+ * environment keeps the defaults.  cnd_tuning reports the choice: shape
+ * index (-1: tiled), segment length L (0: tiled), pencils per workgroup,
+ * segments per pencil in the launch, and the measured milliseconds (0 if
+ * not tuned).                                                              */
+enum { CND_KIND_FILTER = 0, CND_KIND_GRAD, CND_KIND_FLUX, CND_KIND_FIELD, CND_NKIND };
+int cnd_tune(cnd_ctx *ctx);
+/* Pin the shape of one kind and direction (-1: tiled; FP64 only); turns
+ * the automatic tuning off for this context.                               */
+int cnd_set_shape(cnd_ctx *ctx, int kind, int dir, int shape);
+/* Compiled shape number shape (0, 1, ...): segment length, pencils per
+ * workgroup, staging run length, and whether it is a xi (1) or eta/zeta (0)
+ * shape; CND_EINVAL past the last one.                                     */
+int cnd_shape_info(int shape, int *seg_len, int *pencils, int *run, int *xi);
+int cnd_tuning(const cnd_ctx *ctx, int kind, int dir, int *shape, int *seg_len,
+               int *pencils, int *segments, double *ms);
+
+/* Learned closure (no counterpart in the reference): a three-layer
+ * 3x3x3 convolution network, 5 -> 32 -> 32 -> 5 channels with ReLU, on a
+ * periodic n^3 coarse grid, evaluated and trained on the BF16 matrix cores
+ * with FP32 accumulation.  It maps the (normalised) coarse state after a
+ * step to a correction towards the box-filtered fine solution; the compact
+ * filter stays in every step (a case without a network is the plain
+ * scheme).  Parameters: cnd_net_param_count() FP32 values, W1[32][27][5],
+ * b1[32], W2[32][27][32], b2[32], W3[5][27][32], b3[5], tap t =
+ * (dx+1) + 3 (dy+1) + 9 (dz+1).  Coarse blocks have n+1 points per
+ * direction (periodic duplicates).                                         */
+typedef struct cnd_net cnd_net;
+int  cnd_net_param_count(void);
+/* n: coarse cells per direction (multiple of 4); capacity: training pairs
+ * kept on the device; seed: initial weights.                               */
+int  cnd_net_create(int n, int capacity, unsigned long long seed, cnd_net **net);
+void cnd_net_destroy(cnd_net *net);
+const char *cnd_net_error(const cnd_net *net);   /* net may be NULL */
+int  cnd_net_get_params(cnd_net *net, float *w);
+int  cnd_net_set_params(cnd_net *net, const float *w);
+/* Input x = (q - in_mean) / in_scale; correction = y * out_scale (5 each). */
+int  cnd_net_set_norm(cnd_net *net, const double *in_mean, const double *in_scale, const double *out_scale);
+/* Store a training pair: input = state of block bc of coarse, target =
+ * state of block bt of target minus it.  Returns the pair's index (>= 0).  */
+int  cnd_net_add_pair(cnd_net *net, cnd_ctx *coarse, int bc, cnd_ctx *target, int bt);
+/* The same from host arrays in normalised units, [n^3][5] each.           */
+int  cnd_net_add_pair_host(cnd_net *net, const float *x, const float *target);
+int  cnd_net_pairs(const cnd_net *net);
+/* iters Adam steps (learning rate lr) on one stored pair each, drawn with
+ * seed; mean_loss = mean square error over those steps.                    */
+int  cnd_net_train(cnd_net *net, int iters, double lr, unsigned long long seed, double *mean_loss);
+int  cnd_net_eval(cnd_net *net, int first, int count, double *mean_loss);
+/* q += correction(q) on block b of ctx (asynchronous on ctx's stream).     */
+int  cnd_net_apply(cnd_net *net, cnd_ctx *ctx, int b);
+/* Host test hooks, normalised units, channel-last [n^3][5]: forward, and
+ * the gradients of L = sum(gy * y) with respect to x and the parameters.   */
+int  cnd_net_forward_host(cnd_net *net, const float *x, float *y);
+int  cnd_net_grad_host(cnd_net *net, const float *x, const float *gy, float *gx, float *gparams);
+/* State of coarse block bc = top-hat filter (width r = nf / n fine cells,
+ * trapezoidal end weights) of periodic fine block bf sampled at the coarse
+ * points; cubic blocks, r even.                                            */
+int  cnd_box_filter(cnd_ctx *fine, int bf, cnd_ctx *coarse, int bc);
+
 /* Device stream the context works on (a hipStream_t), for callers that
  * order their own work against it.                                         */
 void *cnd_stream(cnd_ctx *ctx);

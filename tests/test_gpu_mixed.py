@@ -69,5 +69,5 @@ def test_mixed_rejects_long_pencils():
     """Mixed precision runs on the segmented sweeps only."""
     from canard_amd.solver import CND_MIXED, Case
     with pytest.raises(RuntimeError, match="segmented"):
-        Case([dict(lxi=400, let=16, lze=16, nbc=[[45, 45]] * 3, mcd=[[0, 0]] * 3)], dict(reoo=100.0),
+        Case([dict(lxi=600, let=16, lze=16, nbc=[[45, 45]] * 3, mcd=[[0, 0]] * 3)], dict(reoo=100.0),
              precision=CND_MIXED)
