@@ -274,7 +274,7 @@ __global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_fd_field(
     const int v = blockIdx.y;
     SrcField<1> src{in + v * in_vstride, 0};
     SinkOut<1> sink{out + v * out_vstride, 0};
-    seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX>(blks, tabs, tb, tiles[tix], d, n45, SP, src, sink, lds);
+    seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, n45, SP, src, sink, lds);
 }
 
 template <class G, bool MIX>
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_filter_field(
     const int v = blockIdx.y;
     SrcField<1> src{in + v * vstride, 0};
     SinkFilt<1> sink{out + v * vstride, in + v * vstride, 0};
-    seg_tile<1, 1, G::L, G::P, G::C, G::XI, MIX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
+    seg_tile<1, 1, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
 }
 
 // Velocity and temperature gradients along d; in mixed mode they are stored
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(4 * G::P * G::SPMAX) void k_seg_grad(
     using TG = typename std::conditional<MIX, float, double>::type;
     SrcPrim src{q, NT};
     SinkOut<4, TG> sink{grad + 4 * d * NT, NT};
-    seg_tile<4, 0, G::L, G::P, G::C, G::XI, MIX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
+    seg_tile<4, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
 }
 
 template <class G, bool MIX>
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(5 * G::P * G::SPMAX) void k_seg_flux(
     const Tile t = tiles[tix];
     SrcFlux<TT> src{q, tau, met, blks, inst, NT, d, vis};
     SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0, d == 0};
-    seg_tile<5, 0, G::L, G::P, G::C, G::XI, MIX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
+    seg_tile<5, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
 }
 
 // ---- pointwise kernels ------------------------------------------------------
@@ -919,6 +919,7 @@ struct cnd_ctx {
     int shape[CND_NKIND][3] = {};
     int seg_sp[CND_NKIND][3] = {};
     bool tuned = false, tune_on = true, tiled_only = false;
+    int lds_max = 65536;   // dynamic LDS per workgroup the device allows
     double tune_ms[CND_NKIND][3] = {};
     std::vector<Tile> hpt;
     Tile *dpt = nullptr;
@@ -1050,7 +1051,8 @@ inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
         const int nt_ = int((c)->hstiles[pi_][d].size());                                                  \
         hipLaunchKernelGGL((K<G, MIX>), dim3(unsigned(xcd_width(nt_)), unsigned(y)),                       \
                            dim3(unsigned((NV) * G::P * sp_)),                                              \
-                           (SegShape<NV, 0, G::L, G::P, G::C, G::XI>::lds_bytes(sp_)), (c)->st, (c)->dblk, \
+                           (SegShape<NV, 0, G::L, G::P, G::C, G::XI>::lds_bytes(sp_, G::SPMAX, (c)->nmax[d])),   \
+                           (c)->st, (c)->dblk,                                                             \
                            (c)->dtab, seg_tabs(c, G::L), (c)->dstiles[pi_][d], nt_, sp_, d, __VA_ARGS__);   \
     } break;
 #define LAUNCH_SEG(K, MIX, NV, c, kind, d, y, ...)                                                          \
@@ -1076,7 +1078,16 @@ inline bool seg_on(cnd_ctx *c, int kind, int d) { return c->shape[kind][d] != kT
 // Segments per pencil a launch of shape sh along d needs (the largest over
 // blocks, rounded up to whole waves per variable where that fits), or 0 when
 // some block's pencils do not fit the shape.
-int shape_sp(cnd_ctx *c, int sh, int d)
+// LDS bytes of a segmented launch (cnd_seg.hpp SegShape::lds_bytes).
+size_t seg_lds_bytes(const ShapeInfo &G, int nv, int sp, int nmax)
+{
+    const size_t chunk = size_t(G.C) * nv * sp * G.P, part = size_t(2) * kSegEndRows * nv * G.P,
+                 ex = size_t(4) * nv * sp * G.P;
+    const size_t work = std::max(std::max(chunk, part), ex);
+    return sizeof(double) * (work + part + size_t(8) * G.SPMAX + size_t(5) * nmax);
+}
+
+int shape_sp(cnd_ctx *c, int sh, int d, int nv)
 {
     const ShapeInfo &G = kShapes[sh];
     if (G.XI != (d == 0)) return 0;
@@ -1089,8 +1100,11 @@ int shape_sp(cnd_ctx *c, int sh, int d)
     if (sp > G.SPMAX) return 0;
     const int per_wave = 64 / G.P;
     const int up = (sp + per_wave - 1) / per_wave * per_wave;
-    return up <= G.SPMAX ? up : sp;
+    sp = up <= G.SPMAX ? up : sp;
+    if (seg_lds_bytes(G, nv, sp, c->nmax[d]) > size_t(c->lds_max)) return 0;
+    return sp;
 }
+inline int kind_nv(int kind) { return kind == CND_KIND_GRAD ? 4 : (kind == CND_KIND_FLUX ? 5 : 1); }
 
 // Filter (kind CND_KIND_FILTER) and plain derivative (CND_KIND_FIELD) sweeps
 // of nvar fields; mix: FP32 solves with FP64 refinement (segmented only).
@@ -1380,25 +1394,30 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         c->tiled_only = sw && std::string(sw) == "tiled";
         const char *tu = std::getenv("CANARD_TUNE");
         c->tune_on = !c->tiled_only && !(tu && std::string(tu) == "0");
-        for (int d = 0; d < 3; ++d) {
-            const int dflt = d == 0 ? 0 : 2;
-            int pick = kTiled;
-            if (!c->tiled_only) {
-                if (shape_sp(c, dflt, d) > 0) pick = dflt;
-                for (int sh = 0; pick == kTiled && sh < kNumShapes; ++sh)
-                    if (shape_sp(c, sh, d) > 0) pick = sh;
-            }
-            for (int k = 0; k < CND_NKIND; ++k) {
-                c->shape[k][d] = pick;
-                c->seg_sp[k][d] = pick == kTiled ? 0 : shape_sp(c, pick, d);
-            }
-            if (precision == CND_MIXED && pick == kTiled) {
-                delete c;
-                return fail(nullptr, CND_EINVAL,
-                            "mixed precision needs the segmented sweeps in every direction (pencils of 13..528 "
-                            "points, and CANARD_SWEEP not set to tiled)");
-            }
+        {
+            int dev = 0, lds = 0;
+            if (hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+                c->lds_max = lds;
         }
+        for (int d = 0; d < 3; ++d)
+            for (int k = 0; k < CND_NKIND; ++k) {
+                const int nv = kind_nv(k), dflt = d == 0 ? 0 : 2;
+                int pick = kTiled;
+                if (!c->tiled_only) {
+                    if (shape_sp(c, dflt, d, nv) > 0) pick = dflt;
+                    for (int sh = 0; pick == kTiled && sh < kNumShapes; ++sh)
+                        if (shape_sp(c, sh, d, nv) > 0) pick = sh;
+                }
+                c->shape[k][d] = pick;
+                c->seg_sp[k][d] = pick == kTiled ? 0 : shape_sp(c, pick, d, nv);
+                if (precision == CND_MIXED && pick == kTiled) {
+                    delete c;
+                    return fail(nullptr, CND_EINVAL,
+                                "mixed precision needs the segmented sweeps in every direction (pencils of 13..528 "
+                                "points, and CANARD_SWEEP not set to tiled)");
+                }
+            }
     }
     // forward values of up to 9 variables per pencil (metric filtering)
     for (int d = 0; d < 3; ++d)
@@ -1668,13 +1687,13 @@ static int tune(cnd_ctx *c)
             std::vector<int> cands;
             if (!mix) cands.push_back(kTiled);
             for (int sh = 0; sh < kNumShapes; ++sh)
-                if (shape_sp(c, sh, d) > 0) cands.push_back(sh);
+                if (shape_sp(c, sh, d, kind_nv(kind)) > 0) cands.push_back(sh);
             if (cands.size() < 2) continue;
             int best = c->shape[kind][d];
             double best_ms = 1e300;
             for (int cand : cands) {
                 c->shape[kind][d] = cand;
-                c->seg_sp[kind][d] = cand == kTiled ? 0 : shape_sp(c, cand, d);
+                c->seg_sp[kind][d] = cand == kTiled ? 0 : shape_sp(c, cand, d, kind_nv(kind));
                 constexpr int reps = 3;
                 if ((r = run(kind, d))) break;
                 (void)hipEventRecord(e0, c->st);
@@ -1689,7 +1708,7 @@ static int tune(cnd_ctx *c)
                 if (ms / reps < best_ms) { best_ms = ms / reps; best = cand; }
             }
             c->shape[kind][d] = best;
-            c->seg_sp[kind][d] = best == kTiled ? 0 : shape_sp(c, best, d);
+            c->seg_sp[kind][d] = best == kTiled ? 0 : shape_sp(c, best, d, kind_nv(kind));
             c->tune_ms[kind][d] = best_ms;
         }
     }
@@ -1726,7 +1745,7 @@ int cnd_set_shape(cnd_ctx *c, int kind, int dir, int shape)
     if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_set_shape: a step is half done");
     if (shape == kTiled && c->precision == CND_MIXED)
         return fail(c, CND_EINVAL, "cnd_set_shape: mixed precision runs on the segmented sweeps only");
-    const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir);
+    const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir, kind_nv(kind));
     if (shape != kTiled && sp == 0)
         return fail(c, CND_EINVAL, "cnd_set_shape: the shape does not fit the pencils of this direction");
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }

@@ -71,9 +71,12 @@ struct SegShape {
         const int m = chunk > part ? chunk : part;
         return m > ex ? m : ex;
     }
-    __host__ __device__ static constexpr size_t lds_bytes(int SP)
+    // then the segment transfer values [SPM][8] and the line's LU rows [n][5]
+    __host__ __device__ static constexpr int ends_doubles() { return 2 * kSegEndRows * NV * P; }
+    __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax)
     {
-        return sizeof(double) * (size_t(work_doubles(SP)) + size_t(2) * kSegEndRows * NV * P);
+        return sizeof(double) * (size_t(work_doubles(SP)) + size_t(ends_doubles()) + size_t(8) * spm +
+                                 size_t(5) * nmax);
     }
 };
 
@@ -91,22 +94,28 @@ struct SegLane {
 };
 
 // Solve the line for the lane's segment held in Y (right-hand side in,
-// solution out) with LU factors LU and homogeneous solutions ST of type T;
-// ex is LDS room for 2 x NV x SP x P doubles.  All threads must call it.
-template <class T, int L>
-__device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const T *__restrict__ LU,
-                                          const T *__restrict__ ST, double *ex)
+// solution out) in arithmetic T.  LUs: the line's LU rows in LDS (5 doubles
+// per row); SM: per-segment transfer values in LDS (phi1, phi2 at the
+// segment's last two rows, psi1, psi2 at its first two); ex: LDS room for
+// 2 x NV x SP x P doubles.  All threads must call it.
+//
+// Each recurrence runs twice over the lane's rows: first with zero incoming
+// values, keeping only the two values that leave the segment; after the scan
+// over the segments has turned those into the true incoming values, again
+// from them, which is the sequential recurrence itself.
+template <class T, int L, int SPM>
+__device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const double *LUs, const double *SM,
+                                          double *ex)
 {
     const int a = ln.a - ln.kb, lo = ln.kb, hi = ln.kb + ln.len, s = ln.s;   // row = a + k, lo <= k < hi
-    // forward recurrence with zero incoming values
+    // forward, pass 1: zero incoming values, outgoing values only
     {
         T ya = T(0), yb = T(0);
 #pragma unroll
         for (int k = 0; k < L; ++k) {
             if (k >= lo && k < hi) {
-                const T *lr = LU + 5 * (a + k);
-                const T y = (Y[k] - lr[0] * ya) - lr[1] * yb;
-                Y[k] = y;
+                const double *lr = LUs + 5 * (a + k);
+                const T y = (Y[k] - T(lr[0]) * ya) - T(lr[1]) * yb;
                 ya = yb;
                 yb = y;
             }
@@ -116,36 +125,40 @@ __device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const T 
         ex[ln.exs + ln.exi] = double(yb);
         __syncthreads();
     }
-    // incoming values of the segment: scan over the segments before it
-    if (ln.ok && s > 0) {
-        T t0 = T(0), t1 = T(0);
+    // incoming values y(a-2), y(a-1): scan over the segments before this one
+    T t0 = T(0), t1 = T(0);
+    if (ln.ok) {
         for (int sp = 0; sp < s; ++sp) {
-            int sa, sl;
-            seg_bounds(ln.n, ln.L, ln.S, sp, sa, sl);
-            const T *m0 = ST + 4 * (sa + sl - 2), *m1 = ST + 4 * (sa + sl - 1);
-            const int xi = ln.exi - (s - sp) * ln.P;
-            const T n0 = (m0[0] * t0 + m0[1] * t1) + T(ex[xi]);
-            const T n1 = (m1[0] * t0 + m1[1] * t1) + T(ex[ln.exs + xi]);
+            const double *m = SM + 8 * sp;
+            const int xi = ln.exi + (sp - s) * ln.P;
+            const T n0 = (T(m[0]) * t0 + T(m[1]) * t1) + T(ex[xi]);
+            const T n1 = (T(m[2]) * t0 + T(m[3]) * t1) + T(ex[ln.exs + xi]);
             t0 = n0;
             t1 = n1;
         }
+    }
+    // forward, pass 2: from the true incoming values
+    {
+        T ya = t0, yb = t1;
 #pragma unroll
         for (int k = 0; k < L; ++k) {
             if (k >= lo && k < hi) {
-                const T *ph = ST + 4 * (a + k);
-                Y[k] = Y[k] + (ph[0] * t0 + ph[1] * t1);
+                const double *lr = LUs + 5 * (a + k);
+                const T y = (Y[k] - T(lr[0]) * ya) - T(lr[1]) * yb;
+                Y[k] = y;
+                ya = yb;
+                yb = y;
             }
         }
     }
-    // backward recurrence with zero incoming values
+    // backward, pass 1
     {
         T xa = T(0), xb = T(0);
 #pragma unroll
         for (int k = L - 1; k >= 0; --k) {
             if (k >= lo && k < hi) {
-                const T *lr = LU + 5 * (a + k);
-                const T x = (lr[2] * Y[k] - lr[3] * xa) - lr[4] * xb;
-                Y[k] = x;
+                const double *lr = LUs + 5 * (a + k);
+                const T x = (T(lr[2]) * Y[k] - T(lr[3]) * xa) - T(lr[4]) * xb;
                 xb = xa;
                 xa = x;
             }
@@ -155,24 +168,29 @@ __device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const T 
         ex[ln.exs + ln.exi] = double(xb);
         __syncthreads();
     }
-    // incoming values from above: scan over the segments after it
-    if (ln.ok && s < ln.S - 1) {
-        T h0 = T(0), h1 = T(0);
+    // incoming values x(b), x(b+1): scan over the segments after this one
+    T h0 = T(0), h1 = T(0);
+    if (ln.ok) {
         for (int sp = ln.S - 1; sp > s; --sp) {
-            int sa, sl;
-            seg_bounds(ln.n, ln.L, ln.S, sp, sa, sl);
-            const T *m0 = ST + 4 * sa, *m1 = ST + 4 * (sa + 1);
+            const double *m = SM + 8 * sp + 4;
             const int xi = ln.exi + (sp - s) * ln.P;
-            const T n0 = (m0[2] * h0 + m0[3] * h1) + T(ex[xi]);
-            const T n1 = (m1[2] * h0 + m1[3] * h1) + T(ex[ln.exs + xi]);
+            const T n0 = (T(m[0]) * h0 + T(m[1]) * h1) + T(ex[xi]);
+            const T n1 = (T(m[2]) * h0 + T(m[3]) * h1) + T(ex[ln.exs + xi]);
             h0 = n0;
             h1 = n1;
         }
+    }
+    // backward, pass 2
+    {
+        T xa = h0, xb = h1;
 #pragma unroll
-        for (int k = 0; k < L; ++k) {
+        for (int k = L - 1; k >= 0; --k) {
             if (k >= lo && k < hi) {
-                const T *ps = ST + 4 * (a + k);
-                Y[k] = Y[k] + (ps[2] * h0 + ps[3] * h1);
+                const double *lr = LUs + 5 * (a + k);
+                const T x = (T(lr[2]) * Y[k] - T(lr[3]) * xa) - T(lr[4]) * xb;
+                Y[k] = x;
+                xb = xa;
+                xa = x;
             }
         }
     }
@@ -191,7 +209,7 @@ struct SegTabs {
 //  Sink: store(g, b, const double (&)[NV]), put(g, b, v, x), ends(side, pencil, v, x)
 // MIX: solve in FP32 and refine once with the FP64 residual r = b - A x
 // (A the banded LHS), x += A^-1 r solved in FP32.
-template <int NV, int SCH, int L, int P, int C, bool XI, bool MIX, class Src, class Sink>
+template <int NV, int SCH, int L, int P, int C, bool XI, bool MIX, int SPM, class Src, class Sink>
 __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
                                          const SegTabs &tb, Tile tile, int d, int n45, int SP, const Src &src,
                                          Sink &sink, double *lds)
@@ -211,6 +229,8 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     const double *ST = tb.seg + 4 * int64_t(B.lu[SCH][d]);
     double *buf = lds;                                  // work area
     double *ends = lds + Sh::work_doubles(SP);         // [2][13][NV][P]
+    double *SM = ends + Sh::ends_doubles();            // [SPM][8]
+    double *LUs = SM + 8 * SPM;                        // [n][5]
     const bool pen_ok = tile.q0 + p < npen;
     const bool lane_ok = pen_ok && s < S;
     int a = 0, len = 0;
@@ -223,6 +243,18 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     };
     auto eidx = [&](int side, int i, int m, int pp) { return ((side * kSegEndRows + i) * NV + m) * P + pp; };
 
+    // ---- 0. the line's LU rows and the segments' transfer values into LDS
+    for (int i = t; i < 5 * n; i += nthr) LUs[i] = LU[i];
+    for (int i = t; i < 8 * S; i += nthr) {
+        const int sp = i / 8, e = i % 8;
+        int sa, sl;
+        seg_bounds(n, L, S, sp, sa, sl);
+        // phi at the last two rows (e 0..3), psi at the first two (e 4..7)
+        const int r = e < 4 ? sa + sl - 2 + (e >> 1) : sa + ((e - 4) >> 1);
+        const int c = e < 4 ? (e & 1) : 2 + (e & 1);
+        SM[i] = (r >= 0 && r < n) ? ST[4 * r + c] : 0.0;
+    }
+
     // ---- 1. stream the segment (with H halo points each side): interior
     //         right-hand sides into registers, end values into LDS
     double Y[L];
@@ -233,23 +265,45 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     for (int m = 0; m <= 2 * H; ++m) w[m] = 0.0;
     const int jfirst = SCH == 0 ? 2 : 3, jlast = SCH == 0 ? N - 2 : N - 3;
     const int nslots = C * SP * P;
+    constexpr int CPT = (C + NV - 1) / NV;   // staging slots per thread and chunk (= C / NV rounded up)
+    // slots whose loads are issued together: as many as keep the loaded
+    // values within about 24 doubles of registers
+    constexpr int RAWD = int(sizeof(typename Src::Raw) / sizeof(double));
+    constexpr int BAT = (24 / RAWD) < 1 ? 1 : ((24 / RAWD) < CPT ? 24 / RAWD : CPT);
 #pragma unroll
     for (int c = 0; c < Sh::NCH; ++c) {
         __syncthreads();
-        for (int slot = t; slot < nslots; slot += nthr) {
-            int o, ss, pp;
-            seg_slot<P, C, XI>(slot, SP, o, ss, pp);
-            if (ss >= S || tile.q0 + pp >= npen) continue;
-            int sa, sl;
-            seg_bounds(n, L, S, ss, sa, sl);
-            const int row = sa - seg_kb(L, S, ss, sl) - H + c * C + o;
-            if (row < 0 || row > N || row < sa - H || row >= sa + sl + H) continue;
-            typename Src::Raw raw;
-            src.load(pt(pp, row), b, raw);
-            double f[NV];
-            src.finish(raw, b, f);
 #pragma unroll
-            for (int m = 0; m < NV; ++m) buf[((o * NV + m) * SP + ss) * P + pp] = f[m];
+        for (int k0 = 0; k0 < CPT; k0 += BAT) {
+        // a group's loads first (addresses clamped into the block, so they are
+        // valid for every slot), then its sources into LDS
+        typename Src::Raw raw[BAT];
+        int dst[BAT];
+#pragma unroll
+        for (int kk = 0; kk < BAT; ++kk) {
+            const int k = k0 + kk;
+            const int slot = k < CPT ? t + k * nthr : nslots;
+            int o, ss, pp;
+            seg_slot<P, C, XI>(slot < nslots ? slot : 0, SP, o, ss, pp);
+            const int ssc = ss < S ? ss : S - 1;
+            int sa, sl;
+            seg_bounds(n, L, S, ssc, sa, sl);
+            const int row = sa - seg_kb(L, S, ssc, sl) - H + c * C + o;
+            const bool ok = slot < nslots && ss < S && tile.q0 + pp < npen && row >= 0 && row <= N &&
+                            row >= sa - H && row < sa + sl + H;
+            const int rowc = row < 0 ? 0 : (row > N ? N : row);
+            const int ppc = tile.q0 + pp < npen ? pp : npen - 1 - tile.q0;
+            src.load(pt(ppc, rowc), b, raw[kk]);
+            dst[kk] = ok ? ((o * NV) * SP + ss) * P + pp : -1;
+        }
+#pragma unroll
+        for (int kk = 0; kk < BAT; ++kk) {
+            if (dst[kk] < 0) continue;
+            double f[NV];
+            src.finish(raw[kk], b, f);
+#pragma unroll
+            for (int m = 0; m < NV; ++m) buf[dst[kk] + m * SP * P] = f[m];
+        }
         }
         __syncthreads();
 #pragma unroll
@@ -420,12 +474,12 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     double *ex = buf;
     const SegLane ln{a, len, kb, s, S, n, L, lane_ok, (v * SP + s) * P + p, NV * SP * P, v, p, P, SP};
     if constexpr (!MIX) {
-        seg_solve<double, L>(Y, ln, LU, ST, ex);
+        seg_solve<double, L, SPM>(Y, ln, LUs, SM, ex);
     } else {
         float X[L];
 #pragma unroll
         for (int k = 0; k < L; ++k) X[k] = float(Y[k]);
-        seg_solve<float, L>(X, ln, tb.lu32 + 5 * int64_t(B.lu[SCH][d]), tb.seg32 + 4 * int64_t(B.lu[SCH][d]), ex);
+        seg_solve<float, L, SPM>(X, ln, LUs, SM, ex);
         // edge values of the neighbouring segments: x(a-2), x(a-1), x(b), x(b+1)
         double l2 = 0.0, l1 = 0.0, r1 = 0.0, r2 = 0.0;
         {
@@ -472,7 +526,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         float D[L];
 #pragma unroll
         for (int k = 0; k < L; ++k) D[k] = float(Y[k]);
-        seg_solve<float, L>(D, ln, tb.lu32 + 5 * int64_t(B.lu[SCH][d]), tb.seg32 + 4 * int64_t(B.lu[SCH][d]), ex);
+        seg_solve<float, L, SPM>(D, ln, LUs, SM, ex);
 #pragma unroll
         for (int k = 0; k < L; ++k) Y[k] = double(X[k]) + double(D[k]);
     }

@@ -89,7 +89,7 @@ def test_autotuned_step_matches_oracle():
     o.close()
 
 
-def test_ensemble_launch_bitwise_equals_separate_contexts():
+def test_ensemble_launch_bitwise_equals_separate_contexts(monkeypatch):
     """Instances of different sizes in one context (one launch per kernel over
     all of them) give bit-for-bit the states of one context per instance with
     the same shapes."""
@@ -98,20 +98,23 @@ def test_ensemble_launch_bitwise_equals_separate_contexts():
     for i, (n, m, re) in enumerate(zip(sizes, machs, res)):
         b, g, s, p = tgv_case(n, mach=m, re=re, pert=1e-3, seed=7 + i)
         b[0]["inst"] = i
+        b[0]["mcd"] = [[i, i]] * 3       # its own periodic partner
         blocks += b; grids += g; states += s; params.append(p)
     from canard_amd.solver import Case
+
+    monkeypatch.setenv("CANARD_TUNE", "0")
+    ens = Case(blocks, params)
+    ens.set_grids(grids); ens.set_states(states); ens.set_clock(inst=0); ens.set_clock(inst=1); ens.set_clock(inst=2)
+    chosen = ens.tuning()
 
     def pin(c):
         for d in range(3):
             for kind in ("filter", "grad", "flux"):
-                c.set_shape(kind, d, 0 if d == 0 else 2)
+                c.set_shape(kind, d, chosen[kind][d]["shape"])
 
-    ens = Case(blocks, params)
-    ens.set_grids(grids); ens.set_states(states); ens.set_clock(inst=0); ens.set_clock(inst=1); ens.set_clock(inst=2)
-    pin(ens)
     ens.step(5)
     for i in range(3):
-        one = _case([dict(blocks[i], inst=0)], [grids[i]], [states[i]], params[i])
+        one = _case([dict(blocks[i], inst=0, mcd=[[0, 0]] * 3)], [grids[i]], [states[i]], params[i])
         pin(one)
         one.step(5)
         assert np.array_equal(one.state(0), ens.state(i)), i
