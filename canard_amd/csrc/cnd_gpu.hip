@@ -1083,7 +1083,8 @@ size_t seg_lds_bytes(const ShapeInfo &G, int nv, int sp, int nmax)
 {
     const size_t chunk = size_t(G.C) * nv * sp * G.P, part = size_t(2) * kSegEndRows * nv * G.P,
                  ex = size_t(4) * nv * sp * G.P;
-    const size_t work = std::max(std::max(chunk, part), ex);
+    const size_t tile = size_t(G.P) * nmax * nv;
+    const size_t work = G.XI ? std::max(tile, ex) : std::max(std::max(chunk, part), ex);
     return sizeof(double) * (work + part + size_t(8) * G.SPMAX + size_t(5) * nmax);
 }
 

@@ -61,7 +61,6 @@ struct SegShape {
     static constexpr int NB = SCH == 0 ? 2 : 3;         // closure rows per end
     static constexpr int KR = L + 2 * H;                // rows streamed per lane
     static constexpr int NCH = (KR + C - 1) / C;        // staging chunks
-    static constexpr int NOUT = (L + C - 1) / C;        // output chunks (xi)
     // LDS: a work area (chunk buffer [C][NV][SP][P], then partner points
     // [2][13][NV][P], then the exchange buffer [4][NV][SP][P]) and the
     // pencil-end values [2][13][NV][P].
@@ -73,10 +72,12 @@ struct SegShape {
     }
     // then the segment transfer values [SPM][8] and the line's LU rows [n][5]
     __host__ __device__ static constexpr int ends_doubles() { return 2 * kSegEndRows * NV * P; }
+    // (xi: the whole tile's sources [nmax][NV][P] replace the work area)
     __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax)
     {
-        return sizeof(double) * (size_t(work_doubles(SP)) + size_t(ends_doubles()) + size_t(8) * spm +
-                                 size_t(5) * nmax);
+        const size_t tile = size_t(P) * nmax * NV, ex4 = size_t(4) * NV * SP * P;
+        const size_t work = XI ? (tile > ex4 ? tile : ex4) : size_t(work_doubles(SP));
+        return sizeof(double) * (work + size_t(ends_doubles()) + size_t(8) * spm + size_t(5) * nmax);
     }
 };
 
@@ -227,10 +228,23 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     const int S = seg_count(n, L);
     const double *LU = tb.lu + 5 * int64_t(B.lu[SCH][d]);
     const double *ST = tb.seg + 4 * int64_t(B.lu[SCH][d]);
-    double *buf = lds;                                  // work area
-    double *ends = lds + Sh::work_doubles(SP);         // [2][13][NV][P]
-    double *SM = ends + Sh::ends_doubles();            // [SPM][8]
-    double *LUs = SM + 8 * SPM;                        // [n][5]
+    // LDS.  eta/zeta: work area (chunk buffer, then partner points, then the
+    // exchange buffer), the pencil-end values, SM, LU rows.  xi: the whole
+    // tile's sources [n][NV][P] (later the exchange buffer and the results),
+    // the partner points, SM, LU rows.
+    double *buf = lds;
+    double *ends = nullptr, *part = nullptr;
+    if constexpr (XI) {
+        const int fsz = P * n * NV, exs4 = 4 * NV * SP * P;
+        part = lds + (fsz > exs4 ? fsz : exs4);
+        ends = part;
+    } else {
+        ends = lds + Sh::work_doubles(SP);
+        part = buf;
+    }
+    double *SM = XI ? part + Sh::ends_doubles() : ends + Sh::ends_doubles();   // [SPM][8]
+    double *LUs = SM + 8 * SPM;                                              // [n][5]
+    auto F = [&](int row) { return buf[(row * NV + v) * P + p]; };          // xi: staged source
     const bool pen_ok = tile.q0 + p < npen;
     const bool lane_ok = pen_ok && s < S;
     int a = 0, len = 0;
@@ -265,6 +279,59 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     for (int m = 0; m <= 2 * H; ++m) w[m] = 0.0;
     const int jfirst = SCH == 0 ? 2 : 3, jlast = SCH == 0 ? N - 2 : N - 3;
     const int nslots = C * SP * P;
+    if constexpr (XI) {
+        // xi pencils are contiguous, and so are the P pencils of a tile: stage
+        // all their points at once, consecutive threads on consecutive points
+        constexpr int RAWX = int(sizeof(typename Src::Raw) / sizeof(double));
+        constexpr int BX = (24 / RAWX) < 1 ? 1 : ((24 / RAWX) > 8 ? 8 : 24 / RAWX);
+        const int tot = P * n;
+        __syncthreads();
+        for (int e0 = t; e0 < tot; e0 += nthr * BX) {
+            typename Src::Raw raw[BX];
+            int dst[BX];
+#pragma unroll
+            for (int kk = 0; kk < BX; ++kk) {
+                const int e = e0 + kk * nthr;
+                const int ec = e < tot ? e : tot - 1;
+                const int pp = ec / n, row = ec % n;
+                const bool ok = e < tot && tile.q0 + pp < npen;
+                const int ppc = tile.q0 + pp < npen ? pp : npen - 1 - tile.q0;
+                src.load(pt(ppc, row), b, raw[kk]);
+                dst[kk] = ok ? row * NV * P + pp : -1;
+            }
+#pragma unroll
+            for (int kk = 0; kk < BX; ++kk) {
+                if (dst[kk] < 0) continue;
+                double f[NV];
+                src.finish(raw[kk], b, f);
+#pragma unroll
+                for (int m = 0; m < NV; ++m) buf[dst[kk] + m * P] = f[m];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kr = 0; kr < Sh::KR; ++kr) {
+            const int row = a - kb - H + kr;
+            const double fv = (lane_ok && row >= 0 && row <= N) ? F(row) : 0.0;
+#pragma unroll
+            for (int m = 0; m < 2 * H; ++m) w[m] = w[m + 1];
+            w[2 * H] = fv;
+            const int k = kr - 2 * H;
+            if (k < 0 || k >= L) continue;
+            const int j = a - kb + k;
+            if (k >= kb && k < kb + len && j >= jfirst && j <= jlast) {
+                double rr;
+                if constexpr (SCH == 0) {
+                    rr = kAa * (w[3] - w[1]) + kAb * (w[4] - w[0]);
+                } else {
+                    const double resk = w[3] + w[3];
+                    rr = (T.fa * ((w[2] + w[4]) - resk) + T.fb * ((w[1] + w[5]) - resk)) +
+                         T.fc * ((w[0] + w[6]) - resk);
+                }
+                Y[k] = rr;
+            }
+        }
+    } else {
     constexpr int CPT = (C + NV - 1) / NV;   // staging slots per thread and chunk (= C / NV rounded up)
     // slots whose loads are issued together: as many as keep the loaded
     // values within about 24 doubles of registers
@@ -336,6 +403,8 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         }
     }
 
+    }
+
     // ---- 2. partner points of interfaces with another block (one round) ------
     bool self[2];
     for (int side = 0; side < 2; ++side) self[side] = B.qd[d][side] != 0 && B.part[d][side] == b;
@@ -358,7 +427,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             double f[NV];
             src.finish(raw, Pb, f);
 #pragma unroll
-            for (int m = 0; m < NV; ++m) buf[eidx(side, o, m, pp)] = f[m];
+            for (int m = 0; m < NV; ++m) part[eidx(side, o, m, pp)] = f[m];
         }
         __syncthreads();
     }
@@ -368,7 +437,8 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         if (!(lane_ok && s == (side == 0 ? 0 : S - 1))) continue;
-        auto f = [&](int i) { return ends[eidx(side, i, v, p)]; };   // distance i from this end
+        // distance i from this end
+        auto f = [&](int i) { return XI ? F(side == 0 ? i : N - i) : ends[eidx(side, i, v, p)]; };
         const bool qd = B.qd[d][side] != 0;
         double rows[3] = {0.0, 0.0, 0.0};
         if (qd) {
@@ -378,8 +448,11 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             const double ra0k = code == 45 ? double(n45) : 0.0;
             // partner point o (o = 12: the partner's face point)
             auto g = [&](int o) {
-                if (self[side]) return ends[eidx(ps, o == kSegEnd ? 0 : o + iik, v, p)];
-                return buf[eidx(side, o, v, p)];
+                if (self[side]) {
+                    const int i = o == kSegEnd ? 0 : o + iik;   // distance from the partner's end ps
+                    return XI ? F(ps == 0 ? i : N - i) : ends[eidx(ps, i, v, p)];
+                }
+                return part[eidx(side, o, v, p)];
             };
             double r0 = 0.0, r1 = 0.0, sap0 = 0.0;
             for (int i = 0; i < kSegEnd; ++i) {
@@ -555,31 +628,25 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
                 if (k >= kb && k < kb + len) sink.put(pb + int64_t(k) * st, b, v, Y[k]);
         }
     } else {
-        // xi: through LDS, stored in runs along the pencil
+        // xi: results into the tile buffer, then stored along the pencils
+        __syncthreads();
+        if (lane_ok) {
 #pragma unroll
-        for (int c = 0; c < Sh::NOUT; ++c) {
-            __syncthreads();
+            for (int k = 0; k < L; ++k)
+                if (k >= kb && k < kb + len) buf[((a - kb + k) * NV + v) * P + p] = Y[k];
+        }
+        __syncthreads();
+        const int tot = P * n;
+        for (int e = t; e < tot; e += nthr) {
+            const int pp = e / n, row = e % n;
+            if (tile.q0 + pp >= npen) continue;
+            double x[NV];
 #pragma unroll
-            for (int o = 0; o < C; ++o) {
-                const int k = c * C + o;
-                if (k < L && k >= kb && k < kb + len) buf[((o * NV + v) * SP + s) * P + p] = Y[k];
-            }
-            __syncthreads();
-            for (int slot = t; slot < nslots; slot += nthr) {
-                int o, ss, pp;
-                seg_slot<P, C, XI>(slot, SP, o, ss, pp);
-                if (ss >= S || tile.q0 + pp >= npen) continue;
-                int sa, sl;
-                seg_bounds(n, L, S, ss, sa, sl);
-                const int skb = seg_kb(L, S, ss, sl), k = c * C + o;
-                if (k < skb || k >= skb + sl) continue;
-                double x[NV];
-#pragma unroll
-                for (int m = 0; m < NV; ++m) x[m] = buf[((o * NV + m) * SP + ss) * P + pp];
-                sink.store(pt(pp, sa - skb + k), b, x);
-            }
+            for (int m = 0; m < NV; ++m) x[m] = buf[(row * NV + m) * P + pp];
+            sink.store(pt(pp, row), b, x);
         }
     }
+
 }
 
 }  // namespace cnd
