@@ -35,12 +35,13 @@ STEP_BYTES_MODEL = 5048.0  # compulsory FP64 bytes per stored point per viscous 
 # (reads + writes of FP64 fields the kernel must touch):
 #   filter sweep (5 variables, one direction): read 5 + write 5 fields
 #   gradient sweep (u,v,w,T along one direction): read q (5) + write 4
-#   stress assembly: read q 5, 12 gradients, 9 metrics + Jacobian, write 9
-#   flux sweep: read q 5, stresses 9, 3 metrics; write de 5 (xi) or
-#               read+write de 5 (eta, zeta) -> mean over the three launches
-#   update: read qo 5, de 5, Jacobian; write qa 5
-SECTION_BYTES = {"filter": 80.0, "grad": 72.0, "stress": 288.0,
-                 "flux": (176.0 + 216.0 + 216.0) / 3.0, "update": 128.0}
+#   stress + flux assembly: read q 5, 12 gradients, 9 metrics + Jacobian;
+#                           write the fluxes of all three directions (15)
+#   flux sweep (one direction): read the 5 fluxes of d, write its own
+#                               right-hand-side buffer (5)
+#   update: read qo 5, the three right-hand-side buffers (15), Jacobian;
+#           write qa 5
+SECTION_BYTES = {"filter": 80.0, "grad": 72.0, "stress": 336.0, "flux": 80.0, "update": 208.0}
 
 
 def parse():

@@ -40,6 +40,8 @@ struct SrcField {   // NV plain fields, vs apart
     const double *a;
     int64_t vs;
     struct Raw { double v[NV]; };
+    static constexpr bool kDirect = true;   // variable v reads field v only
+    __device__ double field(int64_t g, int v) const { return a[v * vs + g]; }
     __device__ void load(int64_t g, int, Raw &r) const
     {
 #pragma unroll
@@ -58,6 +60,8 @@ struct SrcPrim {
     const double *q;
     int64_t NT;
     struct Raw { double q[5]; };
+    static constexpr bool kDirect = false;
+    __device__ double field(int64_t, int) const { return 0.0; }
     __device__ void load(int64_t g, int, Raw &r) const
     {
 #pragma unroll
@@ -74,51 +78,58 @@ struct SrcPrim {
     }
 };
 
-// The five flux components of direction d (src/mo_physics.f90:319-382);
-// TT is the storage type of the stresses (double, or float in mixed mode).
-template <class TT>
+// The five flux components of direction d at one point
+// (src/mo_physics.f90:319-382): the inviscid part from the state q1..q5 and
+// the metric row m of d, then, with stresses t (txx, tyy, tzz, txy, tyz,
+// tzx and the three heat-flux rows), the viscous part.
+template <bool VIS>
+__device__ __forceinline__ void dir_flux(double q1, double q2, double q3, double q4, double q5, double u, double v,
+                                         double w, double p, const double (&m)[3], const double *umf,
+                                         const double *t, double (&o)[5])
+{
+    const double m0 = m[0], m1 = m[1], m2 = m[2];
+    const double s = (m0 * (u + umf[0]) + m1 * (v + umf[1])) + m2 * (w + umf[2]);
+    o[0] = q1 * s;
+    o[1] = q2 * s + m0 * p;
+    o[2] = q3 * s + m1 * p;
+    o[3] = q4 * s + m2 * p;
+    o[4] = (q5 + p) * s - p * ((umf[0] * m0 + umf[1] * m1) + umf[2] * m2);
+    if constexpr (VIS) {
+        // stress rows x (txx, txy, tzx), y (txy, tyy, tyz), z (tzx, tyz, tzz)
+        const double txx = t[0], tyy = t[1], tzz = t[2], txy = t[3], tyz = t[4], tzx = t[5];
+        o[1] = o[1] - m0 * txx; o[1] = o[1] - m1 * txy; o[1] = o[1] - m2 * tzx;
+        o[2] = o[2] - m0 * txy; o[2] = o[2] - m1 * tyy; o[2] = o[2] - m2 * tyz;
+        o[3] = o[3] - m0 * tzx; o[3] = o[3] - m1 * tyz; o[3] = o[3] - m2 * tzz;
+        o[4] = o[4] - m0 * t[6]; o[4] = o[4] - m1 * t[7]; o[4] = o[4] - m2 * t[8];
+    }
+}
+
+// Inviscid fluxes of direction d evaluated in the sweep from q and the
+// metric row of d (viscous runs differentiate the stored fluxes instead).
 struct SrcFlux {
     const double *q;
-    const TT *tau;
     const double *met;
     const BlkDev *blks;
     const InstDev *inst;
     int64_t NT;
-    int d, vis;
-    struct Raw { double q[5], m[3], t[9]; };
+    int d;
+    struct Raw { double q[5], m[3]; };
+    static constexpr bool kDirect = false;
+    __device__ double field(int64_t, int) const { return 0.0; }
     __device__ void load(int64_t g, int, Raw &r) const
     {
 #pragma unroll
         for (int k = 0; k < 5; ++k) r.q[k] = q[k * NT + g];
 #pragma unroll
         for (int k = 0; k < 3; ++k) r.m[k] = met[(3 * d + k) * NT + g];
-        if (vis) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) r.t[k] = double(tau[k * NT + g]);
-        }
     }
     __device__ void finish(const Raw &r, int b, double (&o)[5]) const
     {
-        const double *umf = inst[blks[b].inst].umf;
         const double q1 = r.q[0], q2 = r.q[1], q3 = r.q[2], q4 = r.q[3], q5 = r.q[4];
         const double ri = 1.0 / q1;
         const double u = q2 * ri, v = q3 * ri, w = q4 * ri;
         const double p = kGm1 * (q5 - 0.5 * ((q2 * u + q3 * v) + q4 * w));
-        const double m0 = r.m[0], m1 = r.m[1], m2 = r.m[2];
-        const double s = (m0 * (u + umf[0]) + m1 * (v + umf[1])) + m2 * (w + umf[2]);
-        o[0] = q1 * s;
-        o[1] = q2 * s + m0 * p;
-        o[2] = q3 * s + m1 * p;
-        o[3] = q4 * s + m2 * p;
-        o[4] = (q5 + p) * s - p * ((umf[0] * m0 + umf[1] * m1) + umf[2] * m2);
-        if (vis) {
-            // stress rows x (txx, txy, tzx), y (txy, tyy, tyz), z (tzx, tyz, tzz)
-            const double txx = r.t[0], tyy = r.t[1], tzz = r.t[2], txy = r.t[3], tyz = r.t[4], tzx = r.t[5];
-            o[1] = o[1] - m0 * txx; o[1] = o[1] - m1 * txy; o[1] = o[1] - m2 * tzx;
-            o[2] = o[2] - m0 * txy; o[2] = o[2] - m1 * tyy; o[2] = o[2] - m2 * tyz;
-            o[3] = o[3] - m0 * tzx; o[3] = o[3] - m1 * tyz; o[3] = o[3] - m2 * tzz;
-            o[4] = o[4] - m0 * r.t[6]; o[4] = o[4] - m1 * r.t[7]; o[4] = o[4] - m2 * r.t[8];
-        }
+        dir_flux<false>(q1, q2, q3, q4, q5, u, v, w, p, r.m, inst[blks[b].inst].umf, nullptr, o);
     }
 };
 
@@ -150,17 +161,25 @@ struct SinkFilt {  // out = f + correction
     __device__ void ends(int, int, int, double) {}
 };
 
-struct SinkFlux {  // de = (de +) dF/dxi_d; pencil ends to drva
+struct SinkFlux {  // the five components of dF/dxi_d; pencil ends to drva
     double *de, *drva;
     int64_t NT, FT, fi0;
-    int first;
     __device__ void store(int64_t g, int, const double (&x)[5])
     {
 #pragma unroll
-        for (int m = 0; m < 5; ++m) de[m * NT + g] = first ? x[m] : de[m * NT + g] + x[m];
+        for (int m = 0; m < 5; ++m) de[m * NT + g] = x[m];
     }
-    __device__ void put(int64_t g, int, int m, double x) { de[m * NT + g] = first ? x : de[m * NT + g] + x; }
+    __device__ void put(int64_t g, int, int m, double x) { de[m * NT + g] = x; }
     __device__ void ends(int side, int lane, int m, double x) { drva[(side * 5 + m) * FT + fi0 + lane] = x; }
+};
+
+struct SinkFluxV {  // component v of dF/dxi_d; pencil ends to drva
+    double *o, *drva;
+    int v;
+    int64_t FT, fi0;
+    __device__ void store(int64_t g, int, const double (&x)[1]) { o[g] = x[0]; }
+    __device__ void put(int64_t g, int, int, double x) { o[g] = x; }
+    __device__ void ends(int side, int lane, int, double x) { drva[(side * 5 + v) * FT + fi0 + lane] = x; }
 };
 
 // ---- sweep kernels -----------------------------------------------------------
@@ -213,19 +232,36 @@ __global__ __launch_bounds__(4 * kTile) void k_fd_grad(const BlkDev *__restrict_
     sweep_tile<4, 0, C>(blks, tabs, lu, tiles[tix], tix, d, 0, nmax, ys, src, sink, lds);
 }
 
-// Flux derivative along d accumulated into de (src/mo_physics.f90:385-399).
+// Flux derivatives along d (src/mo_physics.f90:385-399) into the d-th
+// right-hand-side buffer de: inviscid fluxes evaluated in the sweep ...
 template <int C>
 __global__ __launch_bounds__(5 * kTile) void k_fd_flux(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
                                                   const InstDev *__restrict__ inst, const double *__restrict__ lu,
                                                   const Tile *__restrict__ tiles, int ntiles, int d, int nmax,
-                                                  double *ys, const double *q, const double *tau, const double *met,
-                                                  double *de, double *drva, int64_t NT, int64_t FT, int vis)
+                                                  double *ys, const double *q, const double *met, double *de,
+                                                  double *drva, int64_t NT, int64_t FT)
 {
     SWEEP_TILE_PROLOGUE(ntiles)
     const Tile t = tiles[tix];
-    SrcFlux<double> src{q, tau, met, blks, inst, NT, d, vis};
-    SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0, d == 0};
+    SrcFlux src{q, met, blks, inst, NT, d};
+    SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0};
     sweep_tile<5, 0, C>(blks, tabs, lu, t, tix, d, 0, nmax, ys, src, sink, lds);
+}
+
+// ... or stored fluxes F (component v = blockIdx.y), the viscous case.
+template <int C>
+__global__ __launch_bounds__(kTile) void k_fd_flux_field(const BlkDev *__restrict__ blks,
+                                                        const TabDev *__restrict__ tabs,
+                                                        const double *__restrict__ lu, const Tile *__restrict__ tiles,
+                                                        int ntiles, int d, int nmax, double *ys, const double *F,
+                                                        double *de, double *drva, int64_t NT, int64_t FT)
+{
+    SWEEP_TILE_PROLOGUE(ntiles)
+    const int v = blockIdx.y;
+    const Tile t = tiles[tix];
+    SrcField<1> src{F + v * NT, 0};
+    SinkFluxV sink{de + v * NT, drva, v, FT, blks[t.b].foff[d] + t.q0};
+    sweep_tile<1, 0, C>(blks, tabs, lu, t, int64_t(v) * ntiles + tix, d, 0, nmax, ys, src, sink, lds);
 }
 
 // ---- segmented sweeps (cnd_seg.hpp) ---------------------------------------
@@ -237,7 +273,7 @@ __global__ __launch_bounds__(5 * kTile) void k_fd_flux(const BlkDev *__restrict_
 template <int L_, int P_, int C_, int SPMAX_, bool XI_>
 struct SegG { static constexpr int L = L_, P = P_, C = C_, SPMAX = SPMAX_; static constexpr bool XI = XI_; };
 struct ShapeInfo { int L, P, C, SPMAX; bool XI; };
-constexpr int kNumShapes = 7;
+constexpr int kNumShapes = 9;
 constexpr ShapeInfo kShapes[kNumShapes] = {
     {17, 4, 16, 16, true},    // 0: xi default (pencils up to 272 points)
     {33, 4, 16, 8, true},     // 1
@@ -245,7 +281,9 @@ constexpr ShapeInfo kShapes[kNumShapes] = {
     {33, 16, 4, 8, false},    // 3
     {17, 8, 8, 16, false},    // 4
     {33, 4, 16, 16, true},    // 5: long xi pencils (up to 528 points)
-    {33, 8, 8, 16, false}};   // 6: long eta/zeta pencils (up to 528 points)
+    {33, 8, 8, 16, false},    // 6: long eta/zeta pencils (up to 528 points)
+    {17, 8, 16, 16, true},    // 7: xi, wider tiles (single-field sweeps)
+    {17, 16, 16, 16, true}};  // 8
 using Sh0 = SegG<17, 4, 16, 16, true>;
 using Sh1 = SegG<33, 4, 16, 8, true>;
 using Sh2 = SegG<33, 16, 8, 8, false>;
@@ -253,6 +291,11 @@ using Sh3 = SegG<33, 16, 4, 8, false>;
 using Sh4 = SegG<17, 8, 8, 16, false>;
 using Sh5 = SegG<33, 4, 16, 16, true>;
 using Sh6 = SegG<33, 8, 8, 16, false>;
+using Sh7 = SegG<17, 8, 16, 16, true>;
+using Sh8 = SegG<17, 16, 16, 16, true>;
+// workgroup size bound of a kernel (shapes whose NV * P * SPMAX exceeds the
+// 1024-thread limit are rejected at launch time by shape_sp)
+__host__ __device__ constexpr int wg_bound(int x) { return x > 1024 ? 1024 : x; }
 constexpr int kTiled = -1;   // shape index of the tiled sweep
 // pencils-per-workgroup values (tile lists) and segment lengths (tables)
 constexpr int kNumP = 3, kPs[kNumP] = {4, 8, 16};
@@ -266,7 +309,7 @@ __host__ __device__ constexpr int l_index(int L) { return L == 17 ? 0 : 1; }
     extern __shared__ double lds[];
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_fd_field(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX)) void k_seg_fd_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, int n45, const double *in, int64_t in_vstride, double *out, int64_t out_vstride)
 {
@@ -278,7 +321,7 @@ __global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_fd_field(
 }
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_filter_field(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX)) void k_seg_filter_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *in, double *out, int64_t vstride)
 {
@@ -292,7 +335,7 @@ __global__ __launch_bounds__(G::P * G::SPMAX) void k_seg_filter_field(
 // Velocity and temperature gradients along d; in mixed mode they are stored
 // in FP32.
 template <class G, bool MIX>
-__global__ __launch_bounds__(4 * G::P * G::SPMAX) void k_seg_grad(
+__global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX)) void k_seg_grad(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *q, typename std::conditional<MIX, float, double>::type *grad,
     int64_t NT)
@@ -305,18 +348,29 @@ __global__ __launch_bounds__(4 * G::P * G::SPMAX) void k_seg_grad(
 }
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(5 * G::P * G::SPMAX) void k_seg_flux(
+__global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX)) void k_seg_flux(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *q,
-    const typename std::conditional<MIX, float, double>::type *tau, const double *met, double *de, double *drva,
-    int64_t NT, int64_t FT, int vis)
+    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *q, const double *met, double *de,
+    double *drva, int64_t NT, int64_t FT)
 {
     SEG_PROLOGUE(ntiles)
-    using TT = typename std::conditional<MIX, float, double>::type;
     const Tile t = tiles[tix];
-    SrcFlux<TT> src{q, tau, met, blks, inst, NT, d, vis};
-    SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0, d == 0};
+    SrcFlux src{q, met, blks, inst, NT, d};
+    SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0};
     seg_tile<5, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
+}
+
+template <class G, bool MIX>
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX)) void k_seg_flux_field(
+    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
+    int ntiles, int SP, int d, const double *F, double *de, double *drva, int64_t NT, int64_t FT)
+{
+    SEG_PROLOGUE(ntiles)
+    const int v = blockIdx.y;
+    const Tile t = tiles[tix];
+    SrcField<1> src{F + v * NT, 0};
+    SinkFluxV sink{de + v * NT, drva, v, FT, blks[t.b].foff[d] + t.q0};
+    seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
 }
 
 // ---- pointwise kernels ------------------------------------------------------
@@ -328,26 +382,30 @@ __global__ __launch_bounds__(5 * G::P * G::SPMAX) void k_seg_flux(
     if (lp >= npt) return;                                               \
     const int64_t g = B.off + lp;
 
-// Viscous stresses and heat flux (src/mo_physics.f90:245-299).
-// TS: storage type of the gradients and stresses (float in mixed mode).
-template <class TS>
-__global__ __launch_bounds__(kPts) void k_stress(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
-                                                const Tile *__restrict__ tiles, const double *__restrict__ q,
-                                                const TS *__restrict__ grad, const double *__restrict__ met,
-                                                TS *__restrict__ tau, int64_t NT)
+// Viscous stresses and heat flux (src/mo_physics.f90:245-299) and, from
+// them, the fluxes of all three directions (src/mo_physics.f90:319-382),
+// stored as F[5 d + m] for the flux sweeps.  TG: storage type of the
+// gradients (float in mixed mode); stresses and fluxes are FP64.
+template <class TG>
+__global__ __launch_bounds__(kPts) void k_visc_flux(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
+                                                   const Tile *__restrict__ tiles, const double *__restrict__ q,
+                                                   const TG *__restrict__ grad, const double *__restrict__ met,
+                                                   double *__restrict__ F, int64_t NT)
 {
     POINT_PROLOGUE
     const InstDev &I = inst[B.inst];
-    const double ri = 1.0 / q[g];
+    const double q1 = q[g], q5 = q[4 * NT + g];
+    const double ri = 1.0 / q1;
     const double q2 = q[NT + g], q3 = q[2 * NT + g], q4 = q[3 * NT + g];
     const double u = q2 * ri, v = q3 * ri, w = q4 * ri;
-    const double p = kGm1 * (q[4 * NT + g] - 0.5 * ((q2 * u + q3 * v) + q4 * w));
+    const double p = kGm1 * (q5 - 0.5 * ((q2 * u + q3 * v) + q4 * w));
     const double T = (kGam * p) * ri;
     const double mu = (I.srefp1dre * pow(T, 1.5)) / (T + I.srefoo);
+    double mt[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) mt[c] = met[c * NT + g];
     auto G = [&](int d, int var) { return double(grad[(4 * d + var) * NT + g]); };
-    auto CT = [&](int c, int var) {
-        return (met[c * NT + g] * G(0, var) + met[(3 + c) * NT + g] * G(1, var)) + met[(6 + c) * NT + g] * G(2, var);
-    };
+    auto CT = [&](int c, int var) { return (mt[c] * G(0, var) + mt[3 + c] * G(1, var)) + mt[6 + c] * G(2, var); };
     const double vtxx = CT(0, 0), vhzz = CT(1, 0), vtzx = CT(2, 0);
     const double vtxy = CT(0, 1), vtyy = CT(1, 1), vhxx = CT(2, 1);
     const double vhyy = CT(0, 2), vtyz = CT(1, 2), vtzz = CT(2, 2);
@@ -355,13 +413,21 @@ __global__ __launch_bounds__(kPts) void k_stress(const BlkDev *__restrict__ blks
     const double r1 = mu * met[9 * NT + g];
     const double r2 = kGm1PrI * r1;
     const double d5 = kTwoThirds * ((vtxx + vtyy) + vtzz);
+    double ts[9];
     const double txx = r1 * ((vtxx + vtxx) - d5), tyy = r1 * ((vtyy + vtyy) - d5), tzz = r1 * ((vtzz + vtzz) - d5);
     const double txy = r1 * (vtxy + vhzz), tyz = r1 * (vtyz + vhxx), tzx = r1 * (vtzx + vhyy);
-    tau[g] = TS(txx); tau[NT + g] = TS(tyy); tau[2 * NT + g] = TS(tzz);
-    tau[3 * NT + g] = TS(txy); tau[4 * NT + g] = TS(tyz); tau[5 * NT + g] = TS(tzx);
-    tau[6 * NT + g] = TS(((r2 * s1 + u * txx) + v * txy) + w * tzx);
-    tau[7 * NT + g] = TS(((r2 * s2 + u * txy) + v * tyy) + w * tyz);
-    tau[8 * NT + g] = TS(((r2 * s3 + u * tzx) + v * tyz) + w * tzz);
+    ts[0] = txx; ts[1] = tyy; ts[2] = tzz; ts[3] = txy; ts[4] = tyz; ts[5] = tzx;
+    ts[6] = ((r2 * s1 + u * txx) + v * txy) + w * tzx;
+    ts[7] = ((r2 * s2 + u * txy) + v * tyy) + w * tyz;
+    ts[8] = ((r2 * s3 + u * tzx) + v * tyz) + w * tzz;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double m[3] = {mt[3 * d], mt[3 * d + 1], mt[3 * d + 2]};
+        double o[5];
+        dir_flux<true>(q1, q2, q3, q4, q5, u, v, w, p, m, I.umf, ts, o);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) F[(5 * d + c) * NT + g] = o[c];
+    }
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -523,7 +589,7 @@ __global__ __launch_bounds__(kPts) void k_update(const BlkDev *__restrict__ blks
     const double r = dtk * met[9 * NT + g];
     double d[5];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) d[m] = de[m * NT + g];
+    for (int m = 0; m < 5; ++m) d[m] = (de[m * NT + g] + de[(5 + m) * NT + g]) + de[(10 + m) * NT + g];
     if (I.sponge) {
         const double a = sponge[g], bb = sponge[NT + g];
         if (a != 0.0) {
@@ -625,7 +691,8 @@ __global__ __launch_bounds__(kLine) void k_gcbc_setup(const BlkDev *__restrict__
     for (int c = 0; c < 3; ++c) { cm[c] = cmf[(side * 3 + c) * FT + fi]; q24[c] = q[(c + 1) * NT + g]; }
     const Elem e = eleme(cm, q[g], q24, pressure(q, NT, g), inst[B.inst].umf);
     xtq2r(e, cm, xt);
-    for (int c = 0; c < 5; ++c) cha[c] = ra0 * drva[(side * 5 + c) * FT + fi] + ra1 * de[c * NT + g];
+    for (int c = 0; c < 5; ++c)
+        cha[c] = ra0 * drva[(side * 5 + c) * FT + fi] + ra1 * ((de[c * NT + g] + de[(5 + c) * NT + g]) + de[(10 + c) * NT + g]);
     const double y = met[9 * NT + g];
     for (int c = 0; c < 5; ++c) yc[c] = y * cha[c];
     for (int r = 0; r < 5; ++r) {
@@ -705,7 +772,14 @@ __global__ __launch_bounds__(kLine) void k_gcbc_update(const BlkDev *__restrict_
                 }
         const double wv = ii == 0 ? res : res * tabs[B.inst].gcbc_w[ii - 1];
         const double sbcc = wv / double(cnt);
-        for (int c = 0; c < 5; ++c) de[c * NT + gl] = de[c * NT + gl] + sbcc * dha[c];
+        // the right-hand side is the sum of the three direction buffers: the
+        // corrected sum goes to the first, the others are cleared
+        for (int c = 0; c < 5; ++c) {
+            const double tot = (de[c * NT + gl] + de[(5 + c) * NT + gl]) + de[(10 + c) * NT + gl];
+            de[c * NT + gl] = tot + sbcc * dha[c];
+            de[(5 + c) * NT + gl] = 0.0;
+            de[(10 + c) * NT + gl] = 0.0;
+        }
     }
 }
 
@@ -923,14 +997,17 @@ struct cnd_ctx {
     double tune_ms[CND_NKIND][3] = {};
     std::vector<Tile> hpt;
     Tile *dpt = nullptr;
-    double *qa = nullptr, *qo = nullptr, *de = nullptr, *grad = nullptr, *tau = nullptr, *met = nullptr;
+    // de: right-hand side as three 5-field buffers, one per flux direction
+    // (summed in the update); fl: the 15 stored fluxes F[5 d + m] of viscous
+    // stages, scratch for metrics, single sweeps and the autotuner otherwise
+    double *qa = nullptr, *qo = nullptr, *de = nullptr, *grad = nullptr, *fl = nullptr, *met = nullptr;
     double *ys = nullptr, *xyz = nullptr, *sponge = nullptr, *sp_par = nullptr, *tmax = nullptr;
     int nmax[3] = {0, 0, 0};   // longest pencil per direction
     int64_t ys_len = 0;        // doubles in the forward-value scratch
     double *drva[3] = {}, *cmf[3] = {}, *surf[3] = {};
     // mixed precision: FP32 gradients and stresses; FP32 LU / homogeneous
     // tables and the FP64 banded LHS for the refinement
-    float *gradf = nullptr, *tauf = nullptr, *dlu32 = nullptr, *dseg32[kNumL] = {};
+    float *gradf = nullptr, *dlu32 = nullptr, *dseg32[kNumL] = {};
     double *dlhs = nullptr;
     JuncDev jc, je;
     std::vector<char> grid_set;
@@ -1065,6 +1142,8 @@ inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
             SEG_CASE(4, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
             SEG_CASE(5, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
             SEG_CASE(6, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            SEG_CASE(7, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            SEG_CASE(8, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
             default: break;                                                                                \
         }                                                                                                  \
     } while (0)
@@ -1102,10 +1181,16 @@ int shape_sp(cnd_ctx *c, int sh, int d, int nv)
     const int per_wave = 64 / G.P;
     const int up = (sp + per_wave - 1) / per_wave * per_wave;
     sp = up <= G.SPMAX ? up : sp;
+    if (nv * G.P * sp > 1024) return 0;
     if (seg_lds_bytes(G, nv, sp, c->nmax[d]) > size_t(c->lds_max)) return 0;
     return sp;
 }
-inline int kind_nv(int kind) { return kind == CND_KIND_GRAD ? 4 : (kind == CND_KIND_FLUX ? 5 : 1); }
+// Variables per workgroup of a sweep kind (viscous flux sweeps run one
+// workgroup column per component of the stored fluxes).
+inline int kind_nv(const cnd_ctx *c, int kind)
+{
+    return kind == CND_KIND_GRAD ? 4 : (kind == CND_KIND_FLUX && !c->any_visc ? 5 : 1);
+}
 
 // Filter (kind CND_KIND_FILTER) and plain derivative (CND_KIND_FIELD) sweeps
 // of nvar fields; mix: FP32 solves with FP64 refinement (segmented only).
@@ -1150,20 +1235,32 @@ int run_grad(cnd_ctx *c, int d, const double *cur)
     return check_launch(c);
 }
 
-// Flux sweep along d of the state cur into de (kind CND_KIND_FLUX).
+// Flux sweep along d into the d-th right-hand-side buffer (kind
+// CND_KIND_FLUX): of the stored fluxes (viscous runs, one workgroup column
+// per component) or of the inviscid fluxes of the state cur.
 int run_flux(cnd_ctx *c, int d, const double *cur)
 {
     const int K = CND_KIND_FLUX;
-    if (c->precision == CND_MIXED)
-        LAUNCH_SEG(k_seg_flux, true, 5, c, K, d, 1, c->dinst, cur, c->tauf, c->met, c->de, c->drva[d], c->NT,
-                   c->FT[d], int(c->any_visc));
-    else if (seg_on(c, K, d))
-        LAUNCH_SEG(k_seg_flux, false, 5, c, K, d, 1, c->dinst, cur, c->tau, c->met, c->de, c->drva[d], c->NT,
-                   c->FT[d], int(c->any_visc));
-    else
-        LAUNCH_SWEEP(k_fd_flux, 5, c, d, 1, 5 * kTile, c->dblk, c->dtab, c->dinst, c->dlu, c->dtiles[d],
-                     ntiles(c, d), d, c->nmax[d], c->ys, cur, c->tau, c->met, c->de, c->drva[d], c->NT, c->FT[d],
-                     int(c->any_visc));
+    const bool mix = c->precision == CND_MIXED;
+    double *de = c->de + 5 * d * c->NT;
+    if (c->any_visc) {
+        const double *F = c->fl + 5 * d * c->NT;
+        if (mix)
+            LAUNCH_SEG(k_seg_flux_field, true, 1, c, K, d, 5, F, de, c->drva[d], c->NT, c->FT[d]);
+        else if (seg_on(c, K, d))
+            LAUNCH_SEG(k_seg_flux_field, false, 1, c, K, d, 5, F, de, c->drva[d], c->NT, c->FT[d]);
+        else
+            LAUNCH_SWEEP(k_fd_flux_field, 1, c, d, 5, kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d,
+                         c->nmax[d], c->ys, F, de, c->drva[d], c->NT, c->FT[d]);
+    } else {
+        if (mix)
+            LAUNCH_SEG(k_seg_flux, true, 5, c, K, d, 1, c->dinst, cur, c->met, de, c->drva[d], c->NT, c->FT[d]);
+        else if (seg_on(c, K, d))
+            LAUNCH_SEG(k_seg_flux, false, 5, c, K, d, 1, c->dinst, cur, c->met, de, c->drva[d], c->NT, c->FT[d]);
+        else
+            LAUNCH_SWEEP(k_fd_flux, 5, c, d, 1, 5 * kTile, c->dblk, c->dtab, c->dinst, c->dlu, c->dtiles[d],
+                         ntiles(c, d), d, c->nmax[d], c->ys, cur, c->met, de, c->drva[d], c->NT, c->FT[d]);
+    }
     return check_launch(c);
 }
 
@@ -1203,11 +1300,11 @@ int enqueue_stage(cnd_ctx *c, int nk)
     if (c->any_visc) {
         Section sec(c, CND_SEC_STRESS, 1);
         if (c->precision == CND_MIXED)
-            hipLaunchKernelGGL(k_stress<float>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
-                               c->gradf, c->met, c->tauf, c->NT);
+            hipLaunchKernelGGL(k_visc_flux<float>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
+                               c->gradf, c->met, c->fl, c->NT);
         else
-            hipLaunchKernelGGL(k_stress<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
-                               c->grad, c->met, c->tau, c->NT);
+            hipLaunchKernelGGL(k_visc_flux<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt,
+                               cur, c->grad, c->met, c->fl, c->NT);
     }
     {
     Section sec(c, CND_SEC_FLUX, 3);
@@ -1403,7 +1500,7 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         }
         for (int d = 0; d < 3; ++d)
             for (int k = 0; k < CND_NKIND; ++k) {
-                const int nv = kind_nv(k), dflt = d == 0 ? 0 : 2;
+                const int nv = kind_nv(c, k), dflt = d == 0 ? 0 : 2;
                 int pick = kTiled;
                 if (!c->tiled_only) {
                     if (shape_sp(c, dflt, d, nv) > 0) pick = dflt;
@@ -1436,13 +1533,13 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     const int64_t NT = c->NT;
     bool good = ok(dalloc(c, &c->dblk, nblocks)) && ok(dalloc(c, &c->dtab, ninst)) && ok(dalloc(c, &c->dinst, ninst)) &&
                 ok(dalloc(c, &c->dlu, lu.size())) && ok(dalloc(c, &c->qa, 5 * NT)) && ok(dalloc(c, &c->qo, 5 * NT)) &&
-                ok(dalloc(c, &c->de, 5 * NT)) && ok(dalloc(c, &c->grad, 12 * NT)) && ok(dalloc(c, &c->tau, 9 * NT)) &&
+                ok(dalloc(c, &c->de, 15 * NT)) && ok(dalloc(c, &c->grad, 12 * NT)) && ok(dalloc(c, &c->fl, 15 * NT)) &&
                 ok(dalloc(c, &c->met, 10 * NT)) && ok(dalloc(c, &c->ys, c->ys_len)) && ok(dalloc(c, &c->xyz, 3 * NT)) &&
                 ok(dalloc(c, &c->sp_par, 5 * ninst)) && ok(dalloc(c, &c->dseg[0], segtab[0].size())) &&
                 ok(dalloc(c, &c->dseg[1], segtab[1].size()));
     if (good && c->any_sponge) good = ok(dalloc(c, &c->sponge, 2 * NT));
     if (good && precision == CND_MIXED)
-        good = ok(dalloc(c, &c->gradf, 12 * NT)) && ok(dalloc(c, &c->tauf, 9 * NT)) &&
+        good = ok(dalloc(c, &c->gradf, 12 * NT)) &&
                ok(dalloc(c, &c->dlu32, lu.size())) && ok(dalloc(c, &c->dseg32[0], segtab[0].size())) &&
                ok(dalloc(c, &c->dseg32[1], segtab[1].size())) &&
                ok(dalloc(c, &c->dlhs, lhs.size()));
@@ -1496,9 +1593,8 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     up(c->dpt, c->hpt.data(), sizeof(Tile) * c->hpt.size());
     if (!r) r = upload_inst(c);
     if (!r) {
-        hipError_t e = hipMemset(c->de, 0, sizeof(double) * 5 * NT);
-        if (e == hipSuccess) e = hipMemset(c->tau, 0, sizeof(double) * 9 * NT);
-        if (e == hipSuccess && c->tauf) e = hipMemset(c->tauf, 0, sizeof(float) * 9 * NT);
+        hipError_t e = hipMemset(c->de, 0, sizeof(double) * 15 * NT);
+        if (e == hipSuccess) e = hipMemset(c->fl, 0, sizeof(double) * 15 * NT);
         if (e != hipSuccess) r = fail(c, CND_EDEVICE, "memset");
     }
     // junction plans (search_point / search_line)
@@ -1533,8 +1629,8 @@ void cnd_destroy(cnd_ctx *c)
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
-    void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->tau, c->met, c->ys,
-                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->tauf, c->dlu32, c->dseg32[0], c->dseg32[1], c->dlhs, c->jc.tgt, c->jc.src, c->jc.first, c->jc.scale,
+    void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->fl, c->met, c->ys,
+                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->dlu32, c->dseg32[0], c->dseg32[1], c->dlhs, c->jc.tgt, c->jc.src, c->jc.first, c->jc.scale,
                     c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp, c->dseg[0], c->dseg[1]};
     for (void *p : ptrs) if (p) hipFree(p);
     for (int d = 0; d < 3; ++d) {
@@ -1568,16 +1664,16 @@ int cnd_init_metrics(cnd_ctx *c)
         if (!c->grid_set[b]) return fail(c, CND_ESTATE, "cnd_init_metrics: grid of block " + std::to_string(b) + " not set");
     int r;
     // coordinate derivatives with the periodic jump (mpigo n45go + deriv):
-    // dx[(3c+d)] = d(coord c)/d(xi_d) in tau
+    // dx[(3c+d)] = d(coord c)/d(xi_d) in the scratch fl
     for (int d = 0; d < 3; ++d)
-        if ((r = run_fd_field(c, d, 1, c->xyz, c->NT, c->tau + d * c->NT, 3 * c->NT, 3))) return r;
-    hipLaunchKernelGGL(k_cross, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dpt, c->tau, c->met, c->NT);
+        if ((r = run_fd_field(c, d, 1, c->xyz, c->NT, c->fl + d * c->NT, 3 * c->NT, 3))) return r;
+    hipLaunchKernelGGL(k_cross, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dpt, c->fl, c->met, c->NT);
     // filter the nine metric arrays in eta, zeta, xi (mo_grid.f90:117-156)
     if ((r = run_filter_field(c, 1, c->met, c->grad, 9))) return r;
     if ((r = run_filter_field(c, 2, c->grad, c->met, 9))) return r;
     if ((r = run_filter_field(c, 0, c->met, c->grad, 9))) return r;
     hipLaunchKernelGGL(k_copy, dim3(unsigned((9 * c->NT + 255) / 256)), dim3(256), 0, c->st, c->grad, c->met, 9 * c->NT);
-    hipLaunchKernelGGL(k_yaco, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dpt, c->tau, c->met, c->NT);
+    hipLaunchKernelGGL(k_yaco, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dpt, c->fl, c->met, c->NT);
     for (int d = 0; d < 3; ++d)
         hipLaunchKernelGGL(k_normals, tiles_grid(c, d, 2), dim3(kLine), 0, c->st, c->dblk, c->dtiles[d], d, c->met,
                            c->cmf[d], c->NT, c->FT[d]);
@@ -1678,7 +1774,7 @@ static int tune(cnd_ctx *c)
     if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(c, CND_EDEVICE, "tune: event"); }
     int r = CND_OK;
     auto run = [&](int kind, int d) {
-        if (kind == CND_KIND_FILTER) return run_filter_field(c, d, c->qa, c->tau, 5, mix);
+        if (kind == CND_KIND_FILTER) return run_filter_field(c, d, c->qa, c->fl, 5, mix);
         if (kind == CND_KIND_GRAD) return run_grad(c, d, c->qa);
         return run_flux(c, d, c->qa);
     };
@@ -1688,13 +1784,13 @@ static int tune(cnd_ctx *c)
             std::vector<int> cands;
             if (!mix) cands.push_back(kTiled);
             for (int sh = 0; sh < kNumShapes; ++sh)
-                if (shape_sp(c, sh, d, kind_nv(kind)) > 0) cands.push_back(sh);
+                if (shape_sp(c, sh, d, kind_nv(c, kind)) > 0) cands.push_back(sh);
             if (cands.size() < 2) continue;
             int best = c->shape[kind][d];
             double best_ms = 1e300;
             for (int cand : cands) {
                 c->shape[kind][d] = cand;
-                c->seg_sp[kind][d] = cand == kTiled ? 0 : shape_sp(c, cand, d, kind_nv(kind));
+                c->seg_sp[kind][d] = cand == kTiled ? 0 : shape_sp(c, cand, d, kind_nv(c, kind));
                 constexpr int reps = 3;
                 if ((r = run(kind, d))) break;
                 (void)hipEventRecord(e0, c->st);
@@ -1709,7 +1805,7 @@ static int tune(cnd_ctx *c)
                 if (ms / reps < best_ms) { best_ms = ms / reps; best = cand; }
             }
             c->shape[kind][d] = best;
-            c->seg_sp[kind][d] = best == kTiled ? 0 : shape_sp(c, best, d, kind_nv(kind));
+            c->seg_sp[kind][d] = best == kTiled ? 0 : shape_sp(c, best, d, kind_nv(c, kind));
             c->tune_ms[kind][d] = best_ms;
         }
     }
@@ -1746,7 +1842,7 @@ int cnd_set_shape(cnd_ctx *c, int kind, int dir, int shape)
     if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_set_shape: a step is half done");
     if (shape == kTiled && c->precision == CND_MIXED)
         return fail(c, CND_EINVAL, "cnd_set_shape: mixed precision runs on the segmented sweeps only");
-    const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir, kind_nv(kind));
+    const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir, kind_nv(c, kind));
     if (shape != kTiled && sp == 0)
         return fail(c, CND_EINVAL, "cnd_set_shape: the shape does not fit the pencils of this direction");
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -1868,7 +1964,7 @@ int cnd_deriv_sweep(cnd_ctx *c, int dir, const double *const *in, double *const 
 {
     if (!c || !in || !out || dir < 0 || dir > 2) return fail(c, CND_EINVAL, "cnd_deriv_sweep: bad argument");
     int r;
-    double *a = c->tau, *o = c->tau + c->NT;   // scratch fields
+    double *a = c->fl, *o = c->fl + c->NT;   // scratch fields
     if ((r = sweep_io(c, in, a))) return r;
     if ((r = run_fd_field(c, dir, 0, a, 0, o, 0, 1, c->precision == CND_MIXED))) return r;
     return sweep_out(c, o, out);
@@ -1878,7 +1974,7 @@ int cnd_filter_sweep(cnd_ctx *c, int dir, double *const *f)
 {
     if (!c || !f || dir < 0 || dir > 2) return fail(c, CND_EINVAL, "cnd_filter_sweep: bad argument");
     int r;
-    double *a = c->tau, *o = c->tau + c->NT;
+    double *a = c->fl, *o = c->fl + c->NT;   // scratch fields
     if ((r = sweep_io(c, (const double *const *)f, a))) return r;
     if ((r = run_filter_field(c, dir, a, o, 1, c->precision == CND_MIXED))) return r;
     return sweep_out(c, o, f);

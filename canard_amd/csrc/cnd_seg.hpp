@@ -6,23 +6,26 @@
 // and writes each result once (no forward-value scratch in HBM).
 //
 // The LU factors of the line (xl, xu of init_penta/penta) are the ones the
-// sequential sweep uses.  Each lane runs the forward and backward
-// recurrences over its own rows with zero incoming values; the true solution
-// follows by adding homogeneous solutions of the same recurrences restarted
-// at the segment start (forward: phi1, phi2) or end (backward: psi1, psi2),
-// weighted by the two values that enter from the neighbouring segment.
-// Those values are obtained by a short scan over the segments of the pencil
-// through LDS (a 2x2 transfer per segment).  The result equals the
-// sequential recurrence up to rounding.
+// sequential sweep uses.  Each lane runs each recurrence twice over its own
+// rows: first with zero incoming values, which gives the two values the
+// segment would pass on; a short scan over the segments of the pencil
+// through LDS (a 2x2 transfer per segment, built from the homogeneous
+// solutions phi1, phi2 / psi1, psi2 at the segment's ends) turns them into
+// the true incoming values; the second run starts from those and is the
+// sequential recurrence itself, so the result equals it up to rounding.
 //
 // Workgroup: NV * P * SP threads.  Thread t: variable v = t / (P * SP),
-// pencil p = t % P, segment s = (t / P) % SP.  Sources are evaluated
-// cooperatively chunk by chunk (C rows of every segment) into LDS with
-// coalesced loads (xi: C consecutive points of one pencil per run; eta and
-// zeta: P neighbouring pencils per run) and each lane picks its variable.
-// The lanes holding a pencil end keep the 13 source values next to it in LDS;
-// the closure rows are formed from them after the stream (and, for a block
-// that is its own periodic partner, the partner sums too, so no extra loads).
+// pencil p = t % P, segment s = (t / P) % SP.  Sources reach the lanes in
+// one of three ways: xi tiles (P contiguous pencils) are staged whole into
+// LDS with consecutive threads on consecutive points; eta/zeta sweeps of
+// plain fields load each lane's rows directly (the lanes of a wave hold P
+// neighbouring pencils, so every load reads runs of P consecutive points);
+// eta/zeta sweeps of derived sources (primitives, inviscid fluxes) evaluate
+// them cooperatively chunk by chunk (C rows of every segment) into LDS and
+// each lane picks its variable.  The lanes holding a pencil end keep the 13
+// source values next to it in LDS; the closure rows are formed from them
+// after the stream (and, for a block that is its own periodic partner, the
+// partner sums too, so no extra loads).
 #pragma once
 
 #include "cnd_dev.hpp"
@@ -33,6 +36,9 @@ namespace cnd {
 // Points of a pencil end used by the closure rows (12) plus the face point.
 constexpr int kSegEnd = 12;
 constexpr int kSegEndRows = kSegEnd + 1;
+
+// LU values a thread stages per round trip.
+constexpr int kLuBatch = 12;
 
 // Segments of an n-point pencil for lanes of at most L points: 0 when no
 // partition fits (a segment holding a pencil end must contain the 12 points
@@ -258,7 +264,19 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     auto eidx = [&](int side, int i, int m, int pp) { return ((side * kSegEndRows + i) * NV + m) * P + pp; };
 
     // ---- 0. the line's LU rows and the segments' transfer values into LDS
-    for (int i = t; i < 5 * n; i += nthr) LUs[i] = LU[i];
+    //         (a thread's loads are issued together: one round trip per
+    //         kLuBatch * nthr values)
+    for (int i0 = t; i0 < 5 * n; i0 += kLuBatch * nthr) {
+        double lv[kLuBatch];
+#pragma unroll
+        for (int u = 0; u < kLuBatch; ++u) {
+            const int i = i0 + u * nthr;
+            lv[u] = LU[i < 5 * n ? i : 5 * n - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kLuBatch; ++u)
+            if (i0 + u * nthr < 5 * n) LUs[i0 + u * nthr] = lv[u];
+    }
     for (int i = t; i < 8 * S; i += nthr) {
         const int sp = i / 8, e = i % 8;
         int sa, sl;
@@ -283,7 +301,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         // xi pencils are contiguous, and so are the P pencils of a tile: stage
         // all their points at once, consecutive threads on consecutive points
         constexpr int RAWX = int(sizeof(typename Src::Raw) / sizeof(double));
-        constexpr int BX = (24 / RAWX) < 1 ? 1 : ((24 / RAWX) > 8 ? 8 : 24 / RAWX);
+        constexpr int BX = (24 / RAWX) < 1 ? 1 : 24 / RAWX;
         const int tot = P * n;
         __syncthreads();
         for (int e0 = t; e0 < tot; e0 += nthr * BX) {
@@ -327,6 +345,49 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
                     const double resk = w[3] + w[3];
                     rr = (T.fa * ((w[2] + w[4]) - resk) + T.fb * ((w[1] + w[5]) - resk)) +
                          T.fc * ((w[0] + w[6]) - resk);
+                }
+                Y[k] = rr;
+            }
+        }
+    } else if constexpr (Src::kDirect) {
+        // eta/zeta, plain fields: each lane loads its own rows, all at once.
+        // The lanes of a wave hold P neighbouring pencils, so one load
+        // instruction reads runs of P consecutive points.  Rows outside the
+        // lane's window are clamped into it (loaded but never used).
+        double W[Sh::KR];
+        const int ppc = tile.q0 + p < npen ? p : npen - 1 - tile.q0;
+        int64_t pb, st;
+        pencil_of(B, d, tile.q0 + ppc, pb, st);
+        const int r0 = a - kb - H;                 // row held in W[0]
+        const int rlo = (a - H) > 0 ? a - H : 0;
+        const int rhi = (a + len + H - 1) < N ? a + len + H - 1 : N;
+#pragma unroll
+        for (int kr = 0; kr < Sh::KR; ++kr) {
+            int row = r0 + kr;
+            row = row < rlo ? rlo : (row > rhi ? rhi : row);
+            W[kr] = src.field(pb + int64_t(row) * st, v);
+        }
+        // the 13 points next to each pencil end, for the closure rows
+        if (lane_ok && (s == 0 || s == S - 1)) {
+#pragma unroll
+            for (int kr = 0; kr < Sh::KR; ++kr) {
+                const int row = r0 + kr;
+                if (row < rlo || row > rhi) continue;
+                if (s == 0 && row <= kSegEnd) ends[eidx(0, row, v, p)] = W[kr];
+                if (s == S - 1 && N - row <= kSegEnd) ends[eidx(1, N - row, v, p)] = W[kr];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            const int j = a - kb + k;              // its stencil is W[k .. k + 2H]
+            if (k >= kb && k < kb + len && j >= jfirst && j <= jlast) {
+                double rr;
+                if constexpr (SCH == 0) {
+                    rr = kAa * (W[k + 3] - W[k + 1]) + kAb * (W[k + 4] - W[k]);
+                } else {
+                    const double resk = W[k + 3] + W[k + 3];
+                    rr = (T.fa * ((W[k + 2] + W[k + 4]) - resk) + T.fb * ((W[k + 1] + W[k + 5]) - resk)) +
+                         T.fc * ((W[k] + W[k + 6]) - resk);
                 }
                 Y[k] = rr;
             }

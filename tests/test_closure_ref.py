@@ -41,3 +41,17 @@ def test_bf16_rounding():
     assert R.bf16(1.0 + 2 ** -8) == 1.0                    # tie: to even
     assert R.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6      # tie: to even
     assert abs(R.bf16(np.pi) - np.pi) < 2 ** -8 * np.pi
+
+
+def test_closure_norm_stats_share_momentum_scale():
+    from canard_amd.closure_train import norm_stats
+    rng = np.random.default_rng(3)
+    ins = rng.normal(size=(4, 5, 1000))
+    ins[:, 0] += 1.0
+    ins[:, 3] *= 1e-6          # a momentum component that starts near zero
+    outs = rng.normal(size=(4, 5, 1000)) * np.array([1, 2, 2, 2e-6, 3])[None, :, None]
+    m, s, o = norm_stats(ins, outs)
+    assert abs(m[0] - 1.0) < 0.1
+    assert s[1] == s[2] == s[3] and 0.5 < s[1] < 1.0
+    assert o[1] == o[2] == o[3] and 1.0 < o[1] < 2.0
+    assert abs(o[4] - 3.0) < 0.2
