@@ -67,6 +67,9 @@ struct SegShape {
     static constexpr int NB = SCH == 0 ? 2 : 3;         // closure rows per end
     static constexpr int KR = L + 2 * H;                // rows streamed per lane
     static constexpr int NCH = (KR + C - 1) / C;        // staging chunks
+    // xi tiles in LDS: rows of NV * P values padded by one, so that lanes
+    // on consecutive rows of one pencil fall in different banks
+    static constexpr int RS = NV * P + 1;
     // LDS: a work area (chunk buffer [C][NV][SP][P], then partner points
     // [2][13][NV][P], then the exchange buffer [4][NV][SP][P]) and the
     // pencil-end values [2][13][NV][P].
@@ -78,11 +81,13 @@ struct SegShape {
     }
     // then the segment transfer values [SPM][8] and the line's LU rows [n][5]
     __host__ __device__ static constexpr int ends_doubles() { return 2 * kSegEndRows * NV * P; }
-    // (xi: the whole tile's sources [nmax][NV][P] replace the work area)
-    __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax)
+    // (xi: the whole tile's sources [nmax][RS] replace the work area; the
+    // exchange buffer [4][NV][SP][P] overlays the tile, or follows it when
+    // the sink folds the results with the sources)
+    __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax, bool fold)
     {
-        const size_t tile = size_t(P) * nmax * NV, ex4 = size_t(4) * NV * SP * P;
-        const size_t work = XI ? (tile > ex4 ? tile : ex4) : size_t(work_doubles(SP));
+        const size_t tile = size_t(RS) * nmax, ex4 = size_t(4) * NV * SP * P;
+        const size_t work = XI ? (fold ? tile + ex4 : (tile > ex4 ? tile : ex4)) : size_t(work_doubles(SP));
         return sizeof(double) * (work + size_t(ends_doubles()) + size_t(8) * spm + size_t(5) * nmax);
     }
 };
@@ -213,7 +218,10 @@ struct SegTabs {
 
 // One tile of P pencils of block tile.b along d.
 //  Src: load(g, b, Raw&), finish(Raw, b, double (&)[NV])
-//  Sink: store(g, b, const double (&)[NV]), put(g, b, v, x), ends(side, pencil, v, x)
+//  Sink: store(g, b, const double (&)[NV]), put(g, b, v, x), ends(side, pencil, v, x);
+//        fold(f, x): the value to write given the source f and the result x,
+//        store_raw / put_raw: write such values (the sweeps that still hold
+//        the source use these instead of store / put)
 // MIX: solve in FP32 and refine once with the FP64 residual r = b - A x
 // (A the banded LHS), x += A^-1 r solved in FP32.
 template <int NV, int SCH, int L, int P, int C, bool XI, bool MIX, int SPM, class Src, class Sink>
@@ -241,8 +249,8 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     double *buf = lds;
     double *ends = nullptr, *part = nullptr;
     if constexpr (XI) {
-        const int fsz = P * n * NV, exs4 = 4 * NV * SP * P;
-        part = lds + (fsz > exs4 ? fsz : exs4);
+        const int fsz = Sh::RS * n, exs4 = 4 * NV * SP * P;
+        part = lds + (Sink::kFold ? fsz + exs4 : (fsz > exs4 ? fsz : exs4));
         ends = part;
     } else {
         ends = lds + Sh::work_doubles(SP);
@@ -250,7 +258,8 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     }
     double *SM = XI ? part + Sh::ends_doubles() : ends + Sh::ends_doubles();   // [SPM][8]
     double *LUs = SM + 8 * SPM;                                              // [n][5]
-    auto F = [&](int row) { return buf[(row * NV + v) * P + p]; };          // xi: staged source
+    constexpr int RS = Sh::RS;                                              // xi: staged row stride
+    auto F = [&](int row) { return buf[row * RS + v * P + p]; };            // xi: staged source
     const bool pen_ok = tile.q0 + p < npen;
     const bool lane_ok = pen_ok && s < S;
     int a = 0, len = 0;
@@ -292,6 +301,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     double Y[L];
 #pragma unroll
     for (int k = 0; k < L; ++k) Y[k] = 0.0;
+    double W[Sh::KR];   // eta/zeta plain fields: the lane's source rows
     double w[2 * H + 1];
 #pragma unroll
     for (int m = 0; m <= 2 * H; ++m) w[m] = 0.0;
@@ -315,7 +325,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
                 const bool ok = e < tot && tile.q0 + pp < npen;
                 const int ppc = tile.q0 + pp < npen ? pp : npen - 1 - tile.q0;
                 src.load(pt(ppc, row), b, raw[kk]);
-                dst[kk] = ok ? row * NV * P + pp : -1;
+                dst[kk] = ok ? row * RS + pp : -1;
             }
 #pragma unroll
             for (int kk = 0; kk < BX; ++kk) {
@@ -354,7 +364,6 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         // The lanes of a wave hold P neighbouring pencils, so one load
         // instruction reads runs of P consecutive points.  Rows outside the
         // lane's window are clamped into it (loaded but never used).
-        double W[Sh::KR];
         const int ppc = tile.q0 + p < npen ? p : npen - 1 - tile.q0;
         int64_t pb, st;
         pencil_of(B, d, tile.q0 + ppc, pb, st);
@@ -605,7 +614,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     }
 
     // ---- 4. solve: FP64, or FP32 with one FP64 residual refinement ----------
-    double *ex = buf;
+    double *ex = (XI && Sink::kFold) ? buf + Sh::RS * n : buf;
     const SegLane ln{a, len, kb, s, S, n, L, lane_ok, (v * SP + s) * P + p, NV * SP * P, v, p, P, SP};
     if constexpr (!MIX) {
         seg_solve<double, L, SPM>(Y, ln, LUs, SM, ex);
@@ -686,15 +695,25 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             pb += int64_t(a - kb) * st;
 #pragma unroll
             for (int k = 0; k < L; ++k)
-                if (k >= kb && k < kb + len) sink.put(pb + int64_t(k) * st, b, v, Y[k]);
+                if (k >= kb && k < kb + len) {
+                    if constexpr (Src::kDirect)
+                        sink.put_raw(pb + int64_t(k) * st, b, v, sink.fold(W[k + H], Y[k]));
+                    else
+                        sink.put(pb + int64_t(k) * st, b, v, Y[k]);
+                }
         }
     } else {
-        // xi: results into the tile buffer, then stored along the pencils
+        // xi: results (folded with the source where the sink asks for it)
+        // into the tile buffer over the source values, then stored along the
+        // pencils
         __syncthreads();
         if (lane_ok) {
 #pragma unroll
             for (int k = 0; k < L; ++k)
-                if (k >= kb && k < kb + len) buf[((a - kb + k) * NV + v) * P + p] = Y[k];
+                if (k >= kb && k < kb + len) {
+                    double &slot = buf[(a - kb + k) * RS + v * P + p];
+                    slot = sink.fold(slot, Y[k]);
+                }
         }
         __syncthreads();
         const int tot = P * n;
@@ -703,8 +722,8 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             if (tile.q0 + pp >= npen) continue;
             double x[NV];
 #pragma unroll
-            for (int m = 0; m < NV; ++m) x[m] = buf[(row * NV + m) * P + pp];
-            sink.store(pt(pp, row), b, x);
+            for (int m = 0; m < NV; ++m) x[m] = buf[row * RS + m * P + pp];
+            sink.store_raw(pt(pp, row), b, x);
         }
     }
 
