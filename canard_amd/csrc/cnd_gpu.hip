@@ -1191,7 +1191,7 @@ size_t seg_lds_bytes(const ShapeInfo &G, int nv, int sp, int nmax, bool fold)
                  ex = size_t(4) * nv * sp * G.P;
     const size_t tile = size_t(nv * G.P + 1) * nmax;   // padded rows (SegShape::RS)
     const size_t work = G.XI ? (fold ? tile + ex : std::max(tile, ex)) : std::max(std::max(chunk, part), ex);
-    return sizeof(double) * (work + part + size_t(8) * G.SPMAX + size_t(5) * nmax);
+    return sizeof(double) * (work + part + size_t(8) * G.SPMAX + size_t(5) * G.L * G.SPMAX);
 }
 
 // Sweep kinds whose sink folds the results with the sources (the filter).

@@ -37,6 +37,9 @@ namespace cnd {
 constexpr int kSegEnd = 12;
 constexpr int kSegEndRows = kSegEnd + 1;
 
+// Fewest rows of a segment (seg_count) and of a whole pencil (13).
+constexpr int kSegMinRows = 12;
+
 // LU values a thread stages per round trip.
 constexpr int kLuBatch = 12;
 
@@ -46,7 +49,7 @@ constexpr int kLuBatch = 12;
 __host__ __device__ inline int seg_count(int n, int L)
 {
     const int S = (n + L - 1) / L;
-    if (S > 1 && n / S < kSegEnd) return 0;
+    if (S > 1 && n / S < kSegMinRows) return 0;
     return S;
 }
 
@@ -57,9 +60,6 @@ __host__ __device__ inline void seg_bounds(int n, int, int S, int s, int &a, int
     len = q + (s < r ? 1 : 0);
 }
 
-// Register index of a segment's first point: the last segment of a split
-// pencil is right-aligned, so the pencil's last point sits at index L-1.
-__host__ __device__ inline int seg_kb(int L, int S, int s, int len) { return (S > 1 && s == S - 1) ? L - len : 0; }
 
 template <int NV, int SCH, int L, int P, int C, bool XI>
 struct SegShape {
@@ -79,7 +79,8 @@ struct SegShape {
         const int m = chunk > part ? chunk : part;
         return m > ex ? m : ex;
     }
-    // then the segment transfer values [SPM][8] and the line's LU rows [n][5]
+    // then the segment transfer values [SPM][8] and the segments' LU rows as
+    // their lanes hold them [SPM][L][5]
     __host__ __device__ static constexpr int ends_doubles() { return 2 * kSegEndRows * NV * P; }
     // (xi: the whole tile's sources [nmax][RS] replace the work area; the
     // exchange buffer [4][NV][SP][P] overlays the tile, or follows it when
@@ -88,7 +89,7 @@ struct SegShape {
     {
         const size_t tile = size_t(RS) * nmax, ex4 = size_t(4) * NV * SP * P;
         const size_t work = XI ? (fold ? tile + ex4 : (tile > ex4 ? tile : ex4)) : size_t(work_doubles(SP));
-        return sizeof(double) * (work + size_t(ends_doubles()) + size_t(8) * spm + size_t(5) * nmax);
+        return sizeof(double) * (work + size_t(ends_doubles()) + size_t(8) * spm + size_t(5) * L * spm);
     }
 };
 
@@ -100,37 +101,49 @@ __device__ __forceinline__ void seg_slot(int slot, int SP, int &o, int &s, int &
     else { p = slot % P; s = (slot / P) % SP; o = slot / (P * SP); }
 }
 
-// One lane's view of its segment.
+// One lane's view of its segment: rows a .. a + len - 1 of the line, held
+// right-aligned in Y[kb .. L - 1] (kb = L - len); Y[0 .. kb - 1] are
+// pass-through registers.
 struct SegLane {
     int a, len, kb, s, S, n, L, ok, exi, exs, v, p, P, SP;
 };
 
+// LU row of a pass-through register: the forward step y = (Y - l0 ya) - l1 yb
+// with Y = 0 gives y = ya, the backward step x = (l2 Y - l3 xa) - l4 xb
+// gives x = xb; either way the recurrence state (two values) is swapped
+// exactly, and two such rows leave it unchanged.
+__host__ __device__ constexpr double seg_pass_row(int c) { return c == 0 || c == 4 ? -1.0 : 0.0; }
+
 // Solve the line for the lane's segment held in Y (right-hand side in,
-// solution out) in arithmetic T.  LUs: the line's LU rows in LDS (5 doubles
-// per row); SM: per-segment transfer values in LDS (phi1, phi2 at the
-// segment's last two rows, psi1, psi2 at its first two); ex: LDS room for
-// 2 x NV x SP x P doubles.  All threads must call it.
+// solution out; Y[0 .. kb - 1] must be zero) in arithmetic T.  LUs: the
+// LU rows of every segment as the lanes hold them, [SP][L][5] in LDS
+// (pass-through rows in front); SM: per-segment transfer values in LDS
+// (phi1, phi2 at the segment's last two rows, psi1, psi2 at its first two);
+// ex: LDS room for 2 x NV x SP x P doubles.  All threads must call it.
 //
-// Each recurrence runs twice over the lane's rows: first with zero incoming
-// values, keeping only the two values that leave the segment; after the scan
-// over the segments has turned those into the true incoming values, again
-// from them, which is the sequential recurrence itself.
+// Each recurrence runs twice over the lane's registers: first with zero
+// incoming values, keeping only the two values that leave the segment;
+// after the scan over the segments has turned those into the true incoming
+// values, again from them, which is the sequential recurrence itself.  All
+// passes run over all L registers with no branches: the segment's last row
+// is always register L - 1, and the kb pass-through registers carry the
+// state across the front of the segment (swapped kb times, which the start
+// or end values undo).
 template <class T, int L, int SPM>
 __device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const double *LUs, const double *SM,
                                           double *ex)
 {
-    const int a = ln.a - ln.kb, lo = ln.kb, hi = ln.kb + ln.len, s = ln.s;   // row = a + k, lo <= k < hi
+    const int s = ln.s;
+    const bool odd = (ln.kb & 1) != 0;
+    const double *lu = LUs + 5 * L * s;
     // forward, pass 1: zero incoming values, outgoing values only
     {
         T ya = T(0), yb = T(0);
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            if (k >= lo && k < hi) {
-                const double *lr = LUs + 5 * (a + k);
-                const T y = (Y[k] - T(lr[0]) * ya) - T(lr[1]) * yb;
-                ya = yb;
-                yb = y;
-            }
+            const T y = (Y[k] - T(lu[5 * k]) * ya) - T(lu[5 * k + 1]) * yb;
+            ya = yb;
+            yb = y;
         }
         __syncthreads();
         ex[ln.exi] = double(ya);
@@ -151,33 +164,27 @@ __device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const do
     }
     // forward, pass 2: from the true incoming values
     {
-        T ya = t0, yb = t1;
+        T ya = odd ? t1 : t0, yb = odd ? t0 : t1;
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            if (k >= lo && k < hi) {
-                const double *lr = LUs + 5 * (a + k);
-                const T y = (Y[k] - T(lr[0]) * ya) - T(lr[1]) * yb;
-                Y[k] = y;
-                ya = yb;
-                yb = y;
-            }
+            const T y = (Y[k] - T(lu[5 * k]) * ya) - T(lu[5 * k + 1]) * yb;
+            Y[k] = y;
+            ya = yb;
+            yb = y;
         }
     }
-    // backward, pass 1
+    // backward, pass 1: zero incoming values, outgoing values only
     {
         T xa = T(0), xb = T(0);
 #pragma unroll
         for (int k = L - 1; k >= 0; --k) {
-            if (k >= lo && k < hi) {
-                const double *lr = LUs + 5 * (a + k);
-                const T x = (T(lr[2]) * Y[k] - T(lr[3]) * xa) - T(lr[4]) * xb;
-                xb = xa;
-                xa = x;
-            }
+            const T x = (T(lu[5 * k + 2]) * Y[k] - T(lu[5 * k + 3]) * xa) - T(lu[5 * k + 4]) * xb;
+            xb = xa;
+            xa = x;
         }
         __syncthreads();
-        ex[ln.exi] = double(xa);
-        ex[ln.exs + ln.exi] = double(xb);
+        ex[ln.exi] = double(odd ? xb : xa);
+        ex[ln.exs + ln.exi] = double(odd ? xa : xb);
         __syncthreads();
     }
     // incoming values x(b), x(b+1): scan over the segments after this one
@@ -197,15 +204,16 @@ __device__ __forceinline__ void seg_solve(T (&Y)[L], const SegLane &ln, const do
         T xa = h0, xb = h1;
 #pragma unroll
         for (int k = L - 1; k >= 0; --k) {
-            if (k >= lo && k < hi) {
-                const double *lr = LUs + 5 * (a + k);
-                const T x = (T(lr[2]) * Y[k] - T(lr[3]) * xa) - T(lr[4]) * xb;
-                Y[k] = x;
-                xb = xa;
-                xa = x;
-            }
+            const T x = (T(lu[5 * k + 2]) * Y[k] - T(lu[5 * k + 3]) * xa) - T(lu[5 * k + 4]) * xb;
+            Y[k] = x;
+            xb = xa;
+            xa = x;
         }
     }
+    // the solution is complete here: keep the compiler from sinking the
+    // recurrence into the (per-lane conditional) stores that consume it
+#pragma unroll
+    for (int k = 0; k < L; ++k) asm volatile("" : "+v"(Y[k]));
 }
 
 // Coefficient tables of the segmented sweeps, indexed like the LU rows:
@@ -257,14 +265,15 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         part = buf;
     }
     double *SM = XI ? part + Sh::ends_doubles() : ends + Sh::ends_doubles();   // [SPM][8]
-    double *LUs = SM + 8 * SPM;                                              // [n][5]
+    double *LUs = SM + 8 * SPM;                                              // [SP][L][5]
     constexpr int RS = Sh::RS;                                              // xi: staged row stride
     auto F = [&](int row) { return buf[row * RS + v * P + p]; };            // xi: staged source
     const bool pen_ok = tile.q0 + p < npen;
     const bool lane_ok = pen_ok && s < S;
     int a = 0, len = 0;
     if (s < S) seg_bounds(n, L, S, s, a, len);
-    const int kb = seg_kb(L, S, s, len);   // row a + k - kb is held in Y[k], kb <= k < kb + len
+    const int kb = L - len;                 // row a - kb + k is held in Y[k], kb <= k < L
+    auto have = [&](int k) { return k >= kb; };
     auto pt = [&](int pp, int i) -> int64_t {
         int64_t pb, st;
         pencil_of(B, d, tile.q0 + pp, pb, st);
@@ -272,19 +281,28 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     };
     auto eidx = [&](int side, int i, int m, int pp) { return ((side * kSegEndRows + i) * NV + m) * P + pp; };
 
-    // ---- 0. the line's LU rows and the segments' transfer values into LDS
-    //         (a thread's loads are issued together: one round trip per
-    //         kLuBatch * nthr values)
-    for (int i0 = t; i0 < 5 * n; i0 += kLuBatch * nthr) {
-        double lv[kLuBatch];
+    // ---- 0. every segment's LU rows as its lanes hold them ([SP][L][5]:
+    //         pass-through rows, then the segment's rows; a thread's loads are
+    //         issued together, kLuBatch per round trip) and the segments'
+    //         transfer values into LDS
+    {
+        const int nlu = 5 * L * SP, sq = n / (S > 0 ? S : 1), sr = n % (S > 0 ? S : 1);
+        for (int i0 = t; i0 < nlu; i0 += kLuBatch * nthr) {
+            double lv[kLuBatch];
 #pragma unroll
-        for (int u = 0; u < kLuBatch; ++u) {
-            const int i = i0 + u * nthr;
-            lv[u] = LU[i < 5 * n ? i : 5 * n - 1];
+            for (int u = 0; u < kLuBatch; ++u) {
+                const int i = i0 + u * nthr < nlu ? i0 + u * nthr : nlu - 1;
+                const int sg = i / (5 * L), k = (i / 5) % L, c = i % 5;
+                const int sl = sg < S ? sq + (sg < sr ? 1 : 0) : 0;
+                const int sa = sg * sq + (sg < sr ? sg : sr);
+                const bool real = k >= L - sl;
+                const double x = LU[real ? 5 * (sa - (L - sl) + k) + c : 0];
+                lv[u] = real ? x : seg_pass_row(c);
+            }
+#pragma unroll
+            for (int u = 0; u < kLuBatch; ++u)
+                if (i0 + u * nthr < nlu) LUs[i0 + u * nthr] = lv[u];
         }
-#pragma unroll
-        for (int u = 0; u < kLuBatch; ++u)
-            if (i0 + u * nthr < 5 * n) LUs[i0 + u * nthr] = lv[u];
     }
     for (int i = t; i < 8 * S; i += nthr) {
         const int sp = i / 8, e = i % 8;
@@ -305,7 +323,6 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     double w[2 * H + 1];
 #pragma unroll
     for (int m = 0; m <= 2 * H; ++m) w[m] = 0.0;
-    const int jfirst = SCH == 0 ? 2 : 3, jlast = SCH == 0 ? N - 2 : N - 3;
     const int nslots = C * SP * P;
     if constexpr (XI) {
         // xi pencils are contiguous, and so are the P pencils of a tile: stage
@@ -346,18 +363,17 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             w[2 * H] = fv;
             const int k = kr - 2 * H;
             if (k < 0 || k >= L) continue;
-            const int j = a - kb + k;
-            if (k >= kb && k < kb + len && j >= jfirst && j <= jlast) {
-                double rr;
-                if constexpr (SCH == 0) {
-                    rr = kAa * (w[3] - w[1]) + kAb * (w[4] - w[0]);
-                } else {
-                    const double resk = w[3] + w[3];
-                    rr = (T.fa * ((w[2] + w[4]) - resk) + T.fb * ((w[1] + w[5]) - resk)) +
-                         T.fc * ((w[0] + w[6]) - resk);
-                }
-                Y[k] = rr;
+            // interior rows; the closure rows are replaced below and the
+            // pass-through registers cleared
+            double rr;
+            if constexpr (SCH == 0) {
+                rr = kAa * (w[3] - w[1]) + kAb * (w[4] - w[0]);
+            } else {
+                const double resk = w[3] + w[3];
+                rr = (T.fa * ((w[2] + w[4]) - resk) + T.fb * ((w[1] + w[5]) - resk)) +
+                     T.fc * ((w[0] + w[6]) - resk);
             }
+            Y[k] = rr;
         }
     } else if constexpr (Src::kDirect) {
         // eta/zeta, plain fields: each lane loads its own rows, all at once.
@@ -386,20 +402,19 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
                 if (s == S - 1 && N - row <= kSegEnd) ends[eidx(1, N - row, v, p)] = W[kr];
             }
         }
+        // row a - kb + k: stencil W[k .. k + 2H] (interior rows; the closure
+        // rows are replaced below and the pass-through registers cleared)
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            const int j = a - kb + k;              // its stencil is W[k .. k + 2H]
-            if (k >= kb && k < kb + len && j >= jfirst && j <= jlast) {
-                double rr;
-                if constexpr (SCH == 0) {
-                    rr = kAa * (W[k + 3] - W[k + 1]) + kAb * (W[k + 4] - W[k]);
-                } else {
-                    const double resk = W[k + 3] + W[k + 3];
-                    rr = (T.fa * ((W[k + 2] + W[k + 4]) - resk) + T.fb * ((W[k + 1] + W[k + 5]) - resk)) +
-                         T.fc * ((W[k] + W[k + 6]) - resk);
-                }
-                Y[k] = rr;
+            double rr;
+            if constexpr (SCH == 0) {
+                rr = kAa * (W[k + 3] - W[k + 1]) + kAb * (W[k + 4] - W[k]);
+            } else {
+                const double resk = W[k + 3] + W[k + 3];
+                rr = (T.fa * ((W[k + 2] + W[k + 4]) - resk) + T.fb * ((W[k + 1] + W[k + 5]) - resk)) +
+                     T.fc * ((W[k] + W[k + 6]) - resk);
             }
+            Y[k] = rr;
         }
     } else {
     constexpr int CPT = (C + NV - 1) / NV;   // staging slots per thread and chunk (= C / NV rounded up)
@@ -425,7 +440,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             const int ssc = ss < S ? ss : S - 1;
             int sa, sl;
             seg_bounds(n, L, S, ssc, sa, sl);
-            const int row = sa - seg_kb(L, S, ssc, sl) - H + c * C + o;
+            const int row = sa - (L - sl) - H + c * C + o;
             const bool ok = slot < nslots && ss < S && tile.q0 + pp < npen && row >= 0 && row <= N &&
                             row >= sa - H && row < sa + sl + H;
             const int rowc = row < 0 ? 0 : (row > N ? N : row);
@@ -458,22 +473,25 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             w[2 * H] = fv;
             const int k = kr - 2 * H;            // row a - kb + k has its stencil complete
             if (k < 0 || k >= L) continue;
-            const int j = a - kb + k;
-            if (k >= kb && k < kb + len && j >= jfirst && j <= jlast) {
-                double rr;
-                if constexpr (SCH == 0) {
-                    rr = kAa * (w[3] - w[1]) + kAb * (w[4] - w[0]);
-                } else {
-                    const double resk = w[3] + w[3];
-                    rr = (T.fa * ((w[2] + w[4]) - resk) + T.fb * ((w[1] + w[5]) - resk)) +
-                         T.fc * ((w[0] + w[6]) - resk);
-                }
-                Y[k] = rr;
+            double rr;
+            if constexpr (SCH == 0) {
+                rr = kAa * (w[3] - w[1]) + kAb * (w[4] - w[0]);
+            } else {
+                const double resk = w[3] + w[3];
+                rr = (T.fa * ((w[2] + w[4]) - resk) + T.fb * ((w[1] + w[5]) - resk)) +
+                     T.fc * ((w[0] + w[6]) - resk);
             }
+            Y[k] = rr;
         }
     }
 
     }
+
+    // the pass-through registers start at zero (a segment has at least
+    // kSegMinRows rows, so only the first L - kSegMinRows can be such)
+#pragma unroll
+    for (int k = 0; k < L - kSegMinRows; ++k)
+        if (k < kb) Y[k] = 0.0;
 
     // ---- 2. partner points of interfaces with another block (one round) ------
     bool self[2];
@@ -594,23 +612,16 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         for (int r = 0; r < 3; ++r) (side == 0 ? lo : hi)[r] = rows[r];
     }
     // closure rows: rows 0..NB-1 open segment 0; rows N-r close the last one
-    if (lane_ok && s == 0) {
+    if (lane_ok && s == 0) {       // row r is Y[kb + r]
 #pragma unroll
-        for (int r = 0; r < Sh::NB; ++r) Y[r] = lo[r];
+        for (int r = 0; r < Sh::NB; ++r)
+#pragma unroll
+            for (int k = 0; k <= L - kSegMinRows + r; ++k)
+                if (k == kb + r) Y[k] = lo[r];
     }
-    if (lane_ok && s == S - 1) {
-        if (S > 1) {
+    if (lane_ok && s == S - 1) {   // row N is Y[L - 1]
 #pragma unroll
-            for (int r = 0; r < Sh::NB; ++r) Y[L - 1 - r] = hi[r];
-        } else {
-#pragma unroll
-            for (int r = 0; r < Sh::NB; ++r) {
-                const int kk = N - r;
-#pragma unroll
-                for (int k = 0; k < L; ++k)
-                    if (k == kk) Y[k] = hi[r];
-            }
-        }
+        for (int r = 0; r < Sh::NB; ++r) Y[L - 1 - r] = hi[r];
     }
 
     // ---- 4. solve: FP64, or FP32 with one FP64 residual refinement ----------
@@ -631,8 +642,8 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             for (int k = 0; k < L; ++k) {
                 if (k == kb) f0 = X[k];
                 if (k == kb + 1) f1 = X[k];
-                if (k == kb + len - 2) e0 = X[k];
-                if (k == kb + len - 1) e1 = X[k];
+                if (k == L - 2) e0 = X[k];
+                if (k == L - 1) e1 = X[k];
             }
             __syncthreads();
             ex[ln.exi] = f0;
@@ -655,8 +666,8 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         const double *AR = tb.lhs + 5 * int64_t(B.lu[SCH][d]);
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            if (k >= kb && k < kb + len) {
-                const int e = kb + len;
+            if (have(k)) {
+                const int e = L;
                 const double *ar = AR + 5 * (a - kb + k);
                 const double xm2 = k - 2 >= kb ? double(X[k >= 2 ? k - 2 : 0]) : (k - 1 >= kb ? l1 : l2);
                 const double xm1 = k - 1 >= kb ? double(X[k >= 1 ? k - 1 : 0]) : l1;
@@ -673,18 +684,14 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
 #pragma unroll
         for (int k = 0; k < L; ++k) Y[k] = double(X[k]) + double(D[k]);
     }
-    if (lane_ok && s == 0) sink.ends(0, p, v, Y[0]);
-    if (lane_ok && s == S - 1) {
-        double xn = 0.0;
-        if (S > 1) {
-            xn = Y[L - 1];
-        } else {
+    if (lane_ok && s == 0) {
+        double x0 = 0.0;
 #pragma unroll
-            for (int k = 0; k < L; ++k)
-                if (k == N) xn = Y[k];
-        }
-        sink.ends(1, p, v, xn);
+        for (int k = 0; k <= L - kSegMinRows; ++k)
+            if (k == kb) x0 = Y[k];
+        sink.ends(0, p, v, x0);
     }
+    if (lane_ok && s == S - 1) sink.ends(1, p, v, Y[L - 1]);
 
     // ---- 6. results ------------------------------------------------------------
     if constexpr (!XI) {
@@ -695,7 +702,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             pb += int64_t(a - kb) * st;
 #pragma unroll
             for (int k = 0; k < L; ++k)
-                if (k >= kb && k < kb + len) {
+                if (have(k)) {
                     if constexpr (Src::kDirect)
                         sink.put_raw(pb + int64_t(k) * st, b, v, sink.fold(W[k + H], Y[k]));
                     else
@@ -710,7 +717,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         if (lane_ok) {
 #pragma unroll
             for (int k = 0; k < L; ++k)
-                if (k >= kb && k < kb + len) {
+                if (have(k)) {
                     double &slot = buf[(a - kb + k) * RS + v * P + p];
                     slot = sink.fold(slot, Y[k]);
                 }
