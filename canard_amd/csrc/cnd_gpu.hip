@@ -321,6 +321,11 @@ using Sh9 = SegG<17, 16, 8, 16, false>;
 // workgroup size bound of a kernel (shapes whose NV * P * SPMAX exceeds the
 // 1024-thread limit are rejected at launch time by shape_sp)
 __host__ __device__ constexpr int wg_bound(int x) { return x > 1024 ? 1024 : x; }
+// waves per SIMD a segmented sweep is compiled for: xi shapes of 17-point
+// segments are held to 128 VGPRs (4 waves), where their LDS also allows four
+// workgroups of 16 pencils per CU; the others are left to the compiler
+template <class G>
+__host__ __device__ constexpr int seg_waves() { return G::XI && G::L <= 17 ? 4 : 1; }
 constexpr int kTiled = -1;   // shape index of the tiled sweep
 // pencils-per-workgroup values (tile lists) and segment lengths (tables)
 constexpr int kNumP = 3, kPs[kNumP] = {4, 8, 16};
@@ -334,7 +339,7 @@ __host__ __device__ constexpr int l_index(int L) { return L == 17 ? 0 : 1; }
     extern __shared__ double lds[];
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX)) void k_seg_fd_field(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_fd_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, int n45, const double *in, int64_t in_vstride, double *out, int64_t out_vstride)
 {
@@ -346,7 +351,7 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX)) void k_seg_fd_field(
 }
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX)) void k_seg_filter_field(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_filter_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *in, double *out, int64_t vstride)
 {
@@ -360,7 +365,7 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX)) void k_seg_filter_field(
 // Velocity and temperature gradients along d; in mixed mode they are stored
 // in FP32.
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX)) void k_seg_grad(
+__global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX), seg_waves<G>()) void k_seg_grad(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *q, typename std::conditional<MIX, float, double>::type *grad,
     int64_t NT)
@@ -373,7 +378,7 @@ __global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX)) void k_seg_grad(
 }
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX)) void k_seg_flux(
+__global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *q, const double *met, double *de,
     double *drva, int64_t NT, int64_t FT)
@@ -386,7 +391,7 @@ __global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX)) void k_seg_flux(
 }
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX)) void k_seg_flux_field(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *F, double *de, double *drva, int64_t NT, int64_t FT)
 {
@@ -1188,10 +1193,11 @@ inline bool seg_on(cnd_ctx *c, int kind, int d) { return c->shape[kind][d] != kT
 size_t seg_lds_bytes(const ShapeInfo &G, int nv, int sp, int nmax, bool fold)
 {
     const size_t chunk = size_t(G.C) * nv * sp * G.P, part = size_t(2) * kSegEndRows * nv * G.P,
-                 ex = size_t(4) * nv * sp * G.P;
+                 ex = size_t(4) * nv * sp * G.P, lu = size_t(5) * G.L * G.SPMAX;
     const size_t tile = size_t(nv * G.P + 1) * nmax;   // padded rows (SegShape::RS)
-    const size_t work = G.XI ? (fold ? tile + ex : std::max(tile, ex)) : std::max(std::max(chunk, part), ex);
-    return sizeof(double) * (work + part + size_t(8) * G.SPMAX + size_t(5) * G.L * G.SPMAX);
+    const size_t work = G.XI ? (fold ? tile + ex + lu : std::max(tile, lu + ex))
+                             : std::max(std::max(chunk, part), ex) + lu;
+    return sizeof(double) * (work + part + size_t(8) * G.SPMAX);
 }
 
 // Sweep kinds whose sink folds the results with the sources (the filter).

@@ -82,14 +82,17 @@ struct SegShape {
     // then the segment transfer values [SPM][8] and the segments' LU rows as
     // their lanes hold them [SPM][L][5]
     __host__ __device__ static constexpr int ends_doubles() { return 2 * kSegEndRows * NV * P; }
-    // (xi: the whole tile's sources [nmax][RS] replace the work area; the
-    // exchange buffer [4][NV][SP][P] overlays the tile, or follows it when
-    // the sink folds the results with the sources)
+    // (xi: the whole tile's sources [nmax][RS] replace the work area.  When
+    // the sink folds the results with the sources the tile lives to the end
+    // and the exchange buffer [4][NV][SP][P] and the LU rows follow it;
+    // otherwise, once the closure rows are formed, the LU rows and then the
+    // exchange buffer overlay the tile.)
     __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax, bool fold)
     {
-        const size_t tile = size_t(RS) * nmax, ex4 = size_t(4) * NV * SP * P;
-        const size_t work = XI ? (fold ? tile + ex4 : (tile > ex4 ? tile : ex4)) : size_t(work_doubles(SP));
-        return sizeof(double) * (work + size_t(ends_doubles()) + size_t(8) * spm + size_t(5) * L * spm);
+        const size_t tile = size_t(RS) * nmax, ex4 = size_t(4) * NV * SP * P, lu = size_t(5) * L * spm;
+        const size_t over = lu + ex4;
+        const size_t work = XI ? (fold ? tile + ex4 + lu : (tile > over ? tile : over)) : size_t(work_doubles(SP)) + lu;
+        return sizeof(double) * (work + size_t(ends_doubles()) + size_t(8) * spm);
     }
 };
 
@@ -254,18 +257,30 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     // exchange buffer), the pencil-end values, SM, LU rows.  xi: the whole
     // tile's sources [n][NV][P] (later the exchange buffer and the results),
     // the partner points, SM, LU rows.
+    // the LU rows overlay the xi tile (written after the closure rows)
+    constexpr bool LU_OVER = XI && !Sink::kFold;
+    const int nlu = 5 * L * SP;
     double *buf = lds;
-    double *ends = nullptr, *part = nullptr;
+    double *ends = nullptr, *part = nullptr, *LUs = nullptr, *ex = nullptr;
     if constexpr (XI) {
         const int fsz = Sh::RS * n, exs4 = 4 * NV * SP * P;
-        part = lds + (Sink::kFold ? fsz + exs4 : (fsz > exs4 ? fsz : exs4));
+        if constexpr (LU_OVER) {
+            LUs = lds;
+            ex = lds + nlu;
+            part = lds + (fsz > nlu + exs4 ? fsz : nlu + exs4);
+        } else {
+            ex = lds + fsz;
+            LUs = ex + exs4;
+            part = LUs + 5 * L * SPM;
+        }
         ends = part;
     } else {
         ends = lds + Sh::work_doubles(SP);
         part = buf;
+        ex = buf;
+        LUs = ends + Sh::ends_doubles() + 8 * SPM;   // after SM
     }
     double *SM = XI ? part + Sh::ends_doubles() : ends + Sh::ends_doubles();   // [SPM][8]
-    double *LUs = SM + 8 * SPM;                                              // [SP][L][5]
     constexpr int RS = Sh::RS;                                              // xi: staged row stride
     auto F = [&](int row) { return buf[row * RS + v * P + p]; };            // xi: staged source
     const bool pen_ok = tile.q0 + p < npen;
@@ -285,24 +300,41 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     //         pass-through rows, then the segment's rows; a thread's loads are
     //         issued together, kLuBatch per round trip) and the segments'
     //         transfer values into LDS
-    {
-        const int nlu = 5 * L * SP, sq = n / (S > 0 ? S : 1), sr = n % (S > 0 ? S : 1);
-        for (int i0 = t; i0 < nlu; i0 += kLuBatch * nthr) {
-            double lv[kLuBatch];
+    const int sq = n / (S > 0 ? S : 1), sr = n % (S > 0 ? S : 1);
+    auto lu_value = [&](int i) {   // entry i of the [SP][L][5] table
+        const int sg = i / (5 * L), k = (i / 5) % L, c = i % 5;
+        const int sl = sg < S ? sq + (sg < sr ? 1 : 0) : 0;
+        const int sa = sg * sq + (sg < sr ? sg : sr);
+        const bool real = k >= L - sl;
+        const double x = LU[real ? 5 * (sa - (L - sl) + k) + c : 0];
+        return real ? x : seg_pass_row(c);
+    };
+    // LU_OVER: the first KPRE values per thread are loaded once the tile is
+    // staged (their latency overlaps the right-hand sides and closure rows)
+    // and written over the tile afterwards
+    constexpr int KPRE0 = (5 * L + NV * P - 1) / (NV * P);
+    constexpr int KPRE = KPRE0 < kLuBatch ? KPRE0 : kLuBatch;
+    double lv[KPRE];
+    auto lu_fetch = [&]() {
 #pragma unroll
-            for (int u = 0; u < kLuBatch; ++u) {
-                const int i = i0 + u * nthr < nlu ? i0 + u * nthr : nlu - 1;
-                const int sg = i / (5 * L), k = (i / 5) % L, c = i % 5;
-                const int sl = sg < S ? sq + (sg < sr ? 1 : 0) : 0;
-                const int sa = sg * sq + (sg < sr ? sg : sr);
-                const bool real = k >= L - sl;
-                const double x = LU[real ? 5 * (sa - (L - sl) + k) + c : 0];
-                lv[u] = real ? x : seg_pass_row(c);
-            }
+        for (int u = 0; u < KPRE; ++u) lv[u] = lu_value(t + u * nthr < nlu ? t + u * nthr : nlu - 1);
+    };
+    auto lu_store = [&]() {
+#pragma unroll
+        for (int u = 0; u < KPRE; ++u)
+            if (t + u * nthr < nlu) LUs[t + u * nthr] = lv[u];
+        for (int i0 = t + KPRE * nthr; i0 < nlu; i0 += kLuBatch * nthr) {
+            double w2[kLuBatch];
+#pragma unroll
+            for (int u = 0; u < kLuBatch; ++u) w2[u] = lu_value(i0 + u * nthr < nlu ? i0 + u * nthr : nlu - 1);
 #pragma unroll
             for (int u = 0; u < kLuBatch; ++u)
-                if (i0 + u * nthr < nlu) LUs[i0 + u * nthr] = lv[u];
+                if (i0 + u * nthr < nlu) LUs[i0 + u * nthr] = w2[u];
         }
+    };
+    if constexpr (!LU_OVER) {
+        lu_fetch();
+        lu_store();
     }
     for (int i = t; i < 8 * S; i += nthr) {
         const int sp = i / 8, e = i % 8;
@@ -354,6 +386,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             }
         }
         __syncthreads();
+        if constexpr (LU_OVER) lu_fetch();
 #pragma unroll
         for (int kr = 0; kr < Sh::KR; ++kr) {
             const int row = a - kb - H + kr;
@@ -624,8 +657,13 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         for (int r = 0; r < Sh::NB; ++r) Y[L - 1 - r] = hi[r];
     }
 
+    if constexpr (LU_OVER) {   // the tile is no longer read: LU rows over it
+        __syncthreads();
+        lu_store();
+        __syncthreads();
+    }
+
     // ---- 4. solve: FP64, or FP32 with one FP64 residual refinement ----------
-    double *ex = (XI && Sink::kFold) ? buf + Sh::RS * n : buf;
     const SegLane ln{a, len, kb, s, S, n, L, lane_ok, (v * SP + s) * P + p, NV * SP * P, v, p, P, SP};
     if constexpr (!MIX) {
         seg_solve<double, L, SPM>(Y, ln, LUs, SM, ex);
