@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kTile) void k_fd_flux_field(const BlkDev *__restric
 template <int L_, int P_, int C_, int SPMAX_, bool XI_>
 struct SegG { static constexpr int L = L_, P = P_, C = C_, SPMAX = SPMAX_; static constexpr bool XI = XI_; };
 struct ShapeInfo { int L, P, C, SPMAX; bool XI; };
-constexpr int kNumShapes = 10;
+constexpr int kNumShapes = 14;
 constexpr ShapeInfo kShapes[kNumShapes] = {
     {17, 4, 16, 16, true},    // 0: xi default (pencils up to 272 points)
     {33, 4, 16, 8, true},     // 1
@@ -307,7 +307,11 @@ constexpr ShapeInfo kShapes[kNumShapes] = {
     {33, 8, 8, 16, false},    // 6: long eta/zeta pencils (up to 528 points)
     {17, 8, 16, 16, true},    // 7: xi, wider tiles (single-field sweeps)
     {17, 16, 16, 16, true},   // 8
-    {17, 16, 8, 16, false}};  // 9: eta/zeta, short segments, wide tiles
+    {17, 16, 8, 16, false},   // 9: eta/zeta, short segments, wide tiles
+    {33, 32, 8, 8, false},    // 10: eta/zeta, runs of 32 and 64 points
+    {33, 64, 8, 8, false},    // 11
+    {17, 32, 8, 16, false},   // 12
+    {17, 64, 8, 16, false}};  // 13
 using Sh0 = SegG<17, 4, 16, 16, true>;
 using Sh1 = SegG<33, 4, 16, 8, true>;
 using Sh2 = SegG<33, 16, 8, 8, false>;
@@ -318,6 +322,10 @@ using Sh6 = SegG<33, 8, 8, 16, false>;
 using Sh7 = SegG<17, 8, 16, 16, true>;
 using Sh8 = SegG<17, 16, 16, 16, true>;
 using Sh9 = SegG<17, 16, 8, 16, false>;
+using Sh10 = SegG<33, 32, 8, 8, false>;
+using Sh11 = SegG<33, 64, 8, 8, false>;
+using Sh12 = SegG<17, 32, 8, 16, false>;
+using Sh13 = SegG<17, 64, 8, 16, false>;
 // workgroup size bound of a kernel (shapes whose NV * P * SPMAX exceeds the
 // 1024-thread limit are rejected at launch time by shape_sp)
 __host__ __device__ constexpr int wg_bound(int x) { return x > 1024 ? 1024 : x; }
@@ -328,9 +336,9 @@ template <class G>
 __host__ __device__ constexpr int seg_waves() { return G::XI && G::L <= 17 ? 4 : 1; }
 constexpr int kTiled = -1;   // shape index of the tiled sweep
 // pencils-per-workgroup values (tile lists) and segment lengths (tables)
-constexpr int kNumP = 3, kPs[kNumP] = {4, 8, 16};
+constexpr int kNumP = 5, kPs[kNumP] = {4, 8, 16, 32, 64};
 constexpr int kNumL = 2, kLs[kNumL] = {17, 33};
-__host__ __device__ constexpr int p_index(int P) { return P == 4 ? 0 : (P == 8 ? 1 : 2); }
+__host__ __device__ constexpr int p_index(int P) { return P == 4 ? 0 : (P == 8 ? 1 : (P == 16 ? 2 : (P == 32 ? 3 : 4))); }
 __host__ __device__ constexpr int l_index(int L) { return L == 17 ? 0 : 1; }
 
 #define SEG_PROLOGUE(ntiles)               \
@@ -1176,6 +1184,10 @@ inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
             SEG_CASE(7, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
             SEG_CASE(8, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
             SEG_CASE(9, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
+            SEG_CASE(10, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
+            SEG_CASE(11, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
+            SEG_CASE(12, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
+            SEG_CASE(13, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
             default: break;                                                                                \
         }                                                                                                  \
     } while (0)
@@ -1580,10 +1592,10 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
                ok(dalloc(c, &c->dseg32[1], segtab[1].size())) &&
                ok(dalloc(c, &c->dlhs, lhs.size()));
     for (int d = 0; good && d < 3; ++d)
-        good = ok(dalloc(c, &c->dtiles[d], c->htiles[d].size())) && ok(dalloc(c, &c->dstiles[0][d], c->hstiles[0][d].size())) &&
-               ok(dalloc(c, &c->dstiles[1][d], c->hstiles[1][d].size())) &&
-               ok(dalloc(c, &c->dstiles[2][d], c->hstiles[2][d].size())) && ok(dalloc(c, &c->drva[d], 10 * c->FT[d])) &&
+        good = ok(dalloc(c, &c->dtiles[d], c->htiles[d].size())) && ok(dalloc(c, &c->drva[d], 10 * c->FT[d])) &&
                ok(dalloc(c, &c->cmf[d], 6 * c->FT[d])) && ok(dalloc(c, &c->surf[d], 10 * c->FT[d]));
+    for (int d = 0; good && d < 3; ++d)
+        for (int pi = 0; good && pi < kNumP; ++pi) good = ok(dalloc(c, &c->dstiles[pi][d], c->hstiles[pi][d].size()));
     if (good) good = ok(dalloc(c, &c->dpt, c->hpt.size())) && ok(dalloc(c, &c->tmax, 2 * c->hpt.size()));
     if (!good) { std::string m = c->err; cnd_destroy(c); return fail(nullptr, r ? r : CND_ENOMEM, m); }
     std::vector<TabDev> ht(ninst);
