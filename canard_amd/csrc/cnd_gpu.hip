@@ -135,8 +135,8 @@ struct SrcFlux {
 
 // ---- sinks ----------------------------------------------------------------
 // store / put write results; fold(f, x) is the value to write for source f
-// and result x, written with store_raw / put_raw by sweeps that still hold
-// the source (the filter adds its correction to the source).
+// and result x, written with put_raw by sweeps that still hold the source in
+// registers (the filter adds its correction to the source).
 template <int NV, class TO = double>
 struct SinkOut {   // out fields vs apart, stored as TO
     TO *o;
@@ -147,9 +147,7 @@ struct SinkOut {   // out fields vs apart, stored as TO
         for (int k = 0; k < NV; ++k) o[k * vs + g] = TO(x[k]);
     }
     __device__ void put(int64_t g, int, int k, double x) { o[k * vs + g] = TO(x); }
-    static constexpr bool kFold = false;
     __device__ double fold(double, double x) const { return x; }
-    __device__ void store_raw(int64_t g, int b, const double (&x)[NV]) { store(g, b, x); }
     __device__ void put_raw(int64_t g, int b, int k, double x) { put(g, b, k, x); }
     __device__ void ends(int, int, int, double) {}
 };
@@ -165,13 +163,7 @@ struct SinkFilt {  // out = f + correction
         for (int k = 0; k < NV; ++k) o[k * vs + g] = f[k * vs + g] + x[k];
     }
     __device__ void put(int64_t g, int, int k, double x) { o[k * vs + g] = f[k * vs + g] + x; }
-    static constexpr bool kFold = true;
     __device__ double fold(double fv, double x) const { return fv + x; }
-    __device__ void store_raw(int64_t g, int, const double (&x)[NV])
-    {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) o[k * vs + g] = x[k];
-    }
     __device__ void put_raw(int64_t g, int, int k, double x) { o[k * vs + g] = x; }
     __device__ void ends(int, int, int, double) {}
 };
@@ -185,9 +177,7 @@ struct SinkFlux {  // the five components of dF/dxi_d; pencil ends to drva
         for (int m = 0; m < 5; ++m) de[m * NT + g] = x[m];
     }
     __device__ void put(int64_t g, int, int m, double x) { de[m * NT + g] = x; }
-    static constexpr bool kFold = false;
     __device__ double fold(double, double x) const { return x; }
-    __device__ void store_raw(int64_t g, int b, const double (&x)[5]) { store(g, b, x); }
     __device__ void put_raw(int64_t g, int b, int m, double x) { put(g, b, m, x); }
     __device__ void ends(int side, int lane, int m, double x) { drva[(side * 5 + m) * FT + fi0 + lane] = x; }
 };
@@ -198,9 +188,7 @@ struct SinkFluxV {  // component v of dF/dxi_d; pencil ends to drva
     int64_t FT, fi0;
     __device__ void store(int64_t g, int, const double (&x)[1]) { o[g] = x[0]; }
     __device__ void put(int64_t g, int, int, double x) { o[g] = x; }
-    static constexpr bool kFold = false;
     __device__ double fold(double, double x) const { return x; }
-    __device__ void store_raw(int64_t g, int b, const double (&x)[1]) { store(g, b, x); }
     __device__ void put_raw(int64_t g, int b, int k, double x) { put(g, b, k, x); }
     __device__ void ends(int side, int lane, int, double x) { drva[(side * 5 + v) * FT + fi0 + lane] = x; }
 };
@@ -296,7 +284,7 @@ __global__ __launch_bounds__(kTile) void k_fd_flux_field(const BlkDev *__restric
 template <int L_, int P_, int C_, int SPMAX_, bool XI_>
 struct SegG { static constexpr int L = L_, P = P_, C = C_, SPMAX = SPMAX_; static constexpr bool XI = XI_; };
 struct ShapeInfo { int L, P, C, SPMAX; bool XI; };
-constexpr int kNumShapes = 14;
+constexpr int kNumShapes = 15;
 constexpr ShapeInfo kShapes[kNumShapes] = {
     {17, 4, 16, 16, true},    // 0: xi default (pencils up to 272 points)
     {33, 4, 16, 8, true},     // 1
@@ -311,7 +299,8 @@ constexpr ShapeInfo kShapes[kNumShapes] = {
     {33, 32, 8, 8, false},    // 10: eta/zeta, runs of 32 and 64 points
     {33, 64, 8, 8, false},    // 11
     {17, 32, 8, 16, false},   // 12
-    {17, 64, 8, 16, false}};  // 13
+    {17, 64, 8, 16, false},   // 13
+    {17, 32, 16, 16, true}};  // 14: xi, 32-pencil tiles (single-field sweeps)
 using Sh0 = SegG<17, 4, 16, 16, true>;
 using Sh1 = SegG<33, 4, 16, 8, true>;
 using Sh2 = SegG<33, 16, 8, 8, false>;
@@ -326,6 +315,7 @@ using Sh10 = SegG<33, 32, 8, 8, false>;
 using Sh11 = SegG<33, 64, 8, 8, false>;
 using Sh12 = SegG<17, 32, 8, 16, false>;
 using Sh13 = SegG<17, 64, 8, 16, false>;
+using Sh14 = SegG<17, 32, 16, 16, true>;
 // workgroup size bound of a kernel (shapes whose NV * P * SPMAX exceeds the
 // 1024-thread limit are rejected at launch time by shape_sp)
 __host__ __device__ constexpr int wg_bound(int x) { return x > 1024 ? 1024 : x; }
@@ -1166,8 +1156,7 @@ inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
         const int nt_ = int((c)->hstiles[pi_][d].size());                                                  \
         hipLaunchKernelGGL((K<G, MIX>), dim3(unsigned(xcd_width(nt_)), unsigned(y)),                       \
                            dim3(unsigned((NV) * G::P * sp_)),                                              \
-                           (SegShape<NV, 0, G::L, G::P, G::C, G::XI>::lds_bytes(sp_, G::SPMAX, (c)->nmax[d],     \
-                                                                                kind_fold(kind))),                 \
+                           (SegShape<NV, 0, G::L, G::P, G::C, G::XI>::lds_bytes(sp_, G::SPMAX, (c)->nmax[d])),   \
                            (c)->st, (c)->dblk,                                                             \
                            (c)->dtab, seg_tabs(c, G::L), (c)->dstiles[pi_][d], nt_, sp_, d, __VA_ARGS__);   \
     } break;
@@ -1188,6 +1177,7 @@ inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
             SEG_CASE(11, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
             SEG_CASE(12, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
             SEG_CASE(13, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
+            SEG_CASE(14, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
             default: break;                                                                                \
         }                                                                                                  \
     } while (0)
@@ -1202,20 +1192,17 @@ inline bool seg_on(cnd_ctx *c, int kind, int d) { return c->shape[kind][d] != kT
 // blocks, rounded up to whole waves per variable where that fits), or 0 when
 // some block's pencils do not fit the shape.
 // LDS bytes of a segmented launch (cnd_seg.hpp SegShape::lds_bytes).
-size_t seg_lds_bytes(const ShapeInfo &G, int nv, int sp, int nmax, bool fold)
+size_t seg_lds_bytes(const ShapeInfo &G, int nv, int sp, int nmax)
 {
     const size_t chunk = size_t(G.C) * nv * sp * G.P, part = size_t(2) * kSegEndRows * nv * G.P,
                  ex = size_t(4) * nv * sp * G.P, lu = size_t(5) * G.L * G.SPMAX;
     const size_t tile = size_t(nv * G.P + 1) * nmax;   // padded rows (SegShape::RS)
-    const size_t work = G.XI ? (fold ? tile + ex + lu : std::max(tile, lu + ex))
+    const size_t work = G.XI ? std::max(tile, lu + ex)
                              : std::max(std::max(chunk, part), ex) + lu;
     return sizeof(double) * (work + part + size_t(8) * G.SPMAX);
 }
 
-// Sweep kinds whose sink folds the results with the sources (the filter).
-inline bool kind_fold(int kind) { return kind == CND_KIND_FILTER; }
-
-int shape_sp(cnd_ctx *c, int sh, int d, int nv, bool fold)
+int shape_sp(cnd_ctx *c, int sh, int d, int nv)
 {
     const ShapeInfo &G = kShapes[sh];
     if (G.XI != (d == 0)) return 0;
@@ -1230,7 +1217,7 @@ int shape_sp(cnd_ctx *c, int sh, int d, int nv, bool fold)
     const int up = (sp + per_wave - 1) / per_wave * per_wave;
     sp = up <= G.SPMAX ? up : sp;
     if (nv * G.P * sp > 1024) return 0;
-    if (seg_lds_bytes(G, nv, sp, c->nmax[d], fold) > size_t(c->lds_max)) return 0;
+    if (seg_lds_bytes(G, nv, sp, c->nmax[d]) > size_t(c->lds_max)) return 0;
     return sp;
 }
 // Variables per workgroup of a sweep kind (viscous flux sweeps run one
@@ -1551,12 +1538,12 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
                 const int nv = kind_nv(c, k), dflt = d == 0 ? 0 : 2;
                 int pick = kTiled;
                 if (!c->tiled_only) {
-                    if (shape_sp(c, dflt, d, nv, kind_fold(k)) > 0) pick = dflt;
+                    if (shape_sp(c, dflt, d, nv) > 0) pick = dflt;
                     for (int sh = 0; pick == kTiled && sh < kNumShapes; ++sh)
-                        if (shape_sp(c, sh, d, nv, kind_fold(k)) > 0) pick = sh;
+                        if (shape_sp(c, sh, d, nv) > 0) pick = sh;
                 }
                 c->shape[k][d] = pick;
-                c->seg_sp[k][d] = pick == kTiled ? 0 : shape_sp(c, pick, d, nv, kind_fold(k));
+                c->seg_sp[k][d] = pick == kTiled ? 0 : shape_sp(c, pick, d, nv);
                 if (precision == CND_MIXED && pick == kTiled) {
                     delete c;
                     return fail(nullptr, CND_EINVAL,
@@ -1832,13 +1819,13 @@ static int tune(cnd_ctx *c)
             std::vector<int> cands;
             if (!mix) cands.push_back(kTiled);
             for (int sh = 0; sh < kNumShapes; ++sh)
-                if (shape_sp(c, sh, d, kind_nv(c, kind), kind_fold(kind)) > 0) cands.push_back(sh);
+                if (shape_sp(c, sh, d, kind_nv(c, kind)) > 0) cands.push_back(sh);
             if (cands.size() < 2) continue;
             int best = c->shape[kind][d];
             double best_ms = 1e300;
             for (int cand : cands) {
                 c->shape[kind][d] = cand;
-                c->seg_sp[kind][d] = cand == kTiled ? 0 : shape_sp(c, cand, d, kind_nv(c, kind), kind_fold(kind));
+                c->seg_sp[kind][d] = cand == kTiled ? 0 : shape_sp(c, cand, d, kind_nv(c, kind));
                 constexpr int reps = 3;
                 if ((r = run(kind, d))) break;
                 (void)hipEventRecord(e0, c->st);
@@ -1853,7 +1840,7 @@ static int tune(cnd_ctx *c)
                 if (ms / reps < best_ms) { best_ms = ms / reps; best = cand; }
             }
             c->shape[kind][d] = best;
-            c->seg_sp[kind][d] = best == kTiled ? 0 : shape_sp(c, best, d, kind_nv(c, kind), kind_fold(kind));
+            c->seg_sp[kind][d] = best == kTiled ? 0 : shape_sp(c, best, d, kind_nv(c, kind));
             c->tune_ms[kind][d] = best_ms;
         }
     }
@@ -1890,7 +1877,7 @@ int cnd_set_shape(cnd_ctx *c, int kind, int dir, int shape)
     if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_set_shape: a step is half done");
     if (shape == kTiled && c->precision == CND_MIXED)
         return fail(c, CND_EINVAL, "cnd_set_shape: mixed precision runs on the segmented sweeps only");
-    const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir, kind_nv(c, kind), kind_fold(kind));
+    const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir, kind_nv(c, kind));
     if (shape != kTiled && sp == 0)
         return fail(c, CND_EINVAL, "cnd_set_shape: the shape does not fit the pencils of this direction");
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }

@@ -82,16 +82,14 @@ struct SegShape {
     // then the segment transfer values [SPM][8] and the segments' LU rows as
     // their lanes hold them [SPM][L][5]
     __host__ __device__ static constexpr int ends_doubles() { return 2 * kSegEndRows * NV * P; }
-    // (xi: the whole tile's sources [nmax][RS] replace the work area.  When
-    // the sink folds the results with the sources the tile lives to the end
-    // and the exchange buffer [4][NV][SP][P] and the LU rows follow it;
-    // otherwise, once the closure rows are formed, the LU rows and then the
-    // exchange buffer overlay the tile.)
-    __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax, bool fold)
+    // (xi: the whole tile's sources [nmax][RS] replace the work area; once
+    // the closure rows are formed, the LU rows and then the exchange buffer
+    // [4][NV][SP][P] overlay the tile.)
+    __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax)
     {
         const size_t tile = size_t(RS) * nmax, ex4 = size_t(4) * NV * SP * P, lu = size_t(5) * L * spm;
         const size_t over = lu + ex4;
-        const size_t work = XI ? (fold ? tile + ex4 + lu : (tile > over ? tile : over)) : size_t(work_doubles(SP)) + lu;
+        const size_t work = XI ? (tile > over ? tile : over) : size_t(work_doubles(SP)) + lu;
         return sizeof(double) * (work + size_t(ends_doubles()) + size_t(8) * spm);
     }
 };
@@ -231,8 +229,8 @@ struct SegTabs {
 //  Src: load(g, b, Raw&), finish(Raw, b, double (&)[NV])
 //  Sink: store(g, b, const double (&)[NV]), put(g, b, v, x), ends(side, pencil, v, x);
 //        fold(f, x): the value to write given the source f and the result x,
-//        store_raw / put_raw: write such values (the sweeps that still hold
-//        the source use these instead of store / put)
+//        put_raw: write such a value (eta/zeta sweeps of plain fields, which
+//        still hold the source in registers, use these instead of put)
 // MIX: solve in FP32 and refine once with the FP64 residual r = b - A x
 // (A the banded LHS), x += A^-1 r solved in FP32.
 template <int NV, int SCH, int L, int P, int C, bool XI, bool MIX, int SPM, class Src, class Sink>
@@ -258,7 +256,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     // tile's sources [n][NV][P] (later the exchange buffer and the results),
     // the partner points, SM, LU rows.
     // the LU rows overlay the xi tile (written after the closure rows)
-    constexpr bool LU_OVER = XI && !Sink::kFold;
+    constexpr bool LU_OVER = XI;
     const int nlu = 5 * L * SP;
     double *buf = lds;
     double *ends = nullptr, *part = nullptr, *LUs = nullptr, *ex = nullptr;
@@ -748,16 +746,14 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
                 }
         }
     } else {
-        // xi: results (folded with the source where the sink asks for it)
-        // into the tile buffer over the source values, then stored along the
-        // pencils
+        // xi: results into the tile buffer, then stored along the pencils
+        // (the filter's sink adds them to the sources it reads back)
         __syncthreads();
         if (lane_ok) {
 #pragma unroll
             for (int k = 0; k < L; ++k)
                 if (have(k)) {
-                    double &slot = buf[(a - kb + k) * RS + v * P + p];
-                    slot = sink.fold(slot, Y[k]);
+                    buf[(a - kb + k) * RS + v * P + p] = Y[k];
                 }
         }
         __syncthreads();
@@ -768,7 +764,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             double x[NV];
 #pragma unroll
             for (int m = 0; m < NV; ++m) x[m] = buf[row * RS + m * P + pp];
-            sink.store_raw(pt(pp, row), b, x);
+            sink.store(pt(pp, row), b, x);
         }
     }
 
