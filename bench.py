@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same as --n)")
     ap.add_argument("--compare-mixed", type=int, default=1,
                     help="FP64 runs: also time mixed precision and report its error against FP64")
+    ap.add_argument("--shapes", default="", metavar="JSON",
+                    help='pin sweep shapes instead of autotuning, e.g. \'{"grad": [7, 6, 6]}\' '
+                         '(shape index per direction, -1: tiled; see canard_amd.solver.shapes())')
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run 3 device-to-device copies of MIB MiB (a known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
@@ -137,6 +140,9 @@ def main():
         c.set_states([q0])
         c.set_clock()
         c.use_graph(bool(a.graph))
+        for kind, per_dir in (json.loads(a.shapes).items() if a.shapes else []):
+            for d, sh in enumerate(per_dir):
+                c.set_shape(kind, d, int(sh))
         c.step(a.warmup)
         c.synchronize()
         tuning.append(c.tuning())

@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kTile) void k_fd_flux_field(const BlkDev *__restric
 template <int L_, int P_, int C_, int SPMAX_, bool XI_>
 struct SegG { static constexpr int L = L_, P = P_, C = C_, SPMAX = SPMAX_; static constexpr bool XI = XI_; };
 struct ShapeInfo { int L, P, C, SPMAX; bool XI; };
-constexpr int kNumShapes = 15;
+constexpr int kNumShapes = 17;
 constexpr ShapeInfo kShapes[kNumShapes] = {
     {17, 4, 16, 16, true},    // 0: xi default (pencils up to 272 points)
     {33, 4, 16, 8, true},     // 1
@@ -300,7 +300,9 @@ constexpr ShapeInfo kShapes[kNumShapes] = {
     {33, 64, 8, 8, false},    // 11
     {17, 32, 8, 16, false},   // 12
     {17, 64, 8, 16, false},   // 13
-    {17, 32, 16, 16, true}};  // 14: xi, 32-pencil tiles (single-field sweeps)
+    {17, 32, 16, 16, true},   // 14: xi, 32-pencil tiles (single-field sweeps)
+    {17, 20, 16, 16, true},   // 15: xi, tiles sized for three workgroups per CU
+    {17, 24, 16, 16, true}};  // 16
 using Sh0 = SegG<17, 4, 16, 16, true>;
 using Sh1 = SegG<33, 4, 16, 8, true>;
 using Sh2 = SegG<33, 16, 8, 8, false>;
@@ -316,6 +318,8 @@ using Sh11 = SegG<33, 64, 8, 8, false>;
 using Sh12 = SegG<17, 32, 8, 16, false>;
 using Sh13 = SegG<17, 64, 8, 16, false>;
 using Sh14 = SegG<17, 32, 16, 16, true>;
+using Sh15 = SegG<17, 20, 16, 16, true>;
+using Sh16 = SegG<17, 24, 16, 16, true>;
 // workgroup size bound of a kernel (shapes whose NV * P * SPMAX exceeds the
 // 1024-thread limit are rejected at launch time by shape_sp)
 __host__ __device__ constexpr int wg_bound(int x) { return x > 1024 ? 1024 : x; }
@@ -326,9 +330,12 @@ template <class G>
 __host__ __device__ constexpr int seg_waves() { return G::XI && G::L <= 17 ? 4 : 1; }
 constexpr int kTiled = -1;   // shape index of the tiled sweep
 // pencils-per-workgroup values (tile lists) and segment lengths (tables)
-constexpr int kNumP = 5, kPs[kNumP] = {4, 8, 16, 32, 64};
+constexpr int kNumP = 7, kPs[kNumP] = {4, 8, 16, 32, 64, 20, 24};
 constexpr int kNumL = 2, kLs[kNumL] = {17, 33};
-__host__ __device__ constexpr int p_index(int P) { return P == 4 ? 0 : (P == 8 ? 1 : (P == 16 ? 2 : (P == 32 ? 3 : 4))); }
+__host__ __device__ constexpr int p_index(int P)
+{
+    return P == 4 ? 0 : (P == 8 ? 1 : (P == 16 ? 2 : (P == 32 ? 3 : (P == 64 ? 4 : (P == 20 ? 5 : 6)))));
+}
 __host__ __device__ constexpr int l_index(int L) { return L == 17 ? 0 : 1; }
 
 #define SEG_PROLOGUE(ntiles)               \
@@ -1031,6 +1038,7 @@ struct cnd_ctx {
     double *qa = nullptr, *qo = nullptr, *de = nullptr, *grad = nullptr, *fl = nullptr, *met = nullptr;
     double *ys = nullptr, *xyz = nullptr, *sponge = nullptr, *sp_par = nullptr, *tmax = nullptr;
     int nmax[3] = {0, 0, 0};   // longest pencil per direction
+    bool part_other[3] = {};   // some block has another block as interface partner along d
     int64_t ys_len = 0;        // doubles in the forward-value scratch
     double *drva[3] = {}, *cmf[3] = {}, *surf[3] = {};
     // mixed precision: FP32 gradients and stresses; FP32 LU / homogeneous
@@ -1156,7 +1164,8 @@ inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
         const int nt_ = int((c)->hstiles[pi_][d].size());                                                  \
         hipLaunchKernelGGL((K<G, MIX>), dim3(unsigned(xcd_width(nt_)), unsigned(y)),                       \
                            dim3(unsigned((NV) * G::P * sp_)),                                              \
-                           (SegShape<NV, 0, G::L, G::P, G::C, G::XI>::lds_bytes(sp_, G::SPMAX, (c)->nmax[d])),   \
+                           (SegShape<NV, 0, G::L, G::P, G::C, G::XI>::lds_bytes(sp_, G::SPMAX, (c)->nmax[d],     \
+                                                                                (c)->part_other[d])),              \
                            (c)->st, (c)->dblk,                                                             \
                            (c)->dtab, seg_tabs(c, G::L), (c)->dstiles[pi_][d], nt_, sp_, d, __VA_ARGS__);   \
     } break;
@@ -1178,6 +1187,8 @@ inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
             SEG_CASE(12, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
             SEG_CASE(13, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
             SEG_CASE(14, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
+            SEG_CASE(15, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
+            SEG_CASE(16, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
             default: break;                                                                                \
         }                                                                                                  \
     } while (0)
@@ -1192,14 +1203,13 @@ inline bool seg_on(cnd_ctx *c, int kind, int d) { return c->shape[kind][d] != kT
 // blocks, rounded up to whole waves per variable where that fits), or 0 when
 // some block's pencils do not fit the shape.
 // LDS bytes of a segmented launch (cnd_seg.hpp SegShape::lds_bytes).
-size_t seg_lds_bytes(const ShapeInfo &G, int nv, int sp, int nmax)
+size_t seg_lds_bytes(const ShapeInfo &G, int nv, int sp, int nmax, bool has_part)
 {
     const size_t chunk = size_t(G.C) * nv * sp * G.P, part = size_t(2) * kSegEndRows * nv * G.P,
                  ex = size_t(4) * nv * sp * G.P, lu = size_t(5) * G.L * G.SPMAX;
     const size_t tile = size_t(nv * G.P + 1) * nmax;   // padded rows (SegShape::RS)
-    const size_t work = G.XI ? std::max(tile, lu + ex)
-                             : std::max(std::max(chunk, part), ex) + lu;
-    return sizeof(double) * (work + part + size_t(8) * G.SPMAX);
+    if (G.XI) return sizeof(double) * (std::max(tile, lu + ex) + size_t(8) * G.SPMAX + (has_part ? part : 0));
+    return sizeof(double) * (std::max(std::max(chunk, part), ex) + lu + part + size_t(8) * G.SPMAX);
 }
 
 int shape_sp(cnd_ctx *c, int sh, int d, int nv)
@@ -1217,7 +1227,7 @@ int shape_sp(cnd_ctx *c, int sh, int d, int nv)
     const int up = (sp + per_wave - 1) / per_wave * per_wave;
     sp = up <= G.SPMAX ? up : sp;
     if (nv * G.P * sp > 1024) return 0;
-    if (seg_lds_bytes(G, nv, sp, c->nmax[d]) > size_t(c->lds_max)) return 0;
+    if (seg_lds_bytes(G, nv, sp, c->nmax[d], c->part_other[d]) > size_t(c->lds_max)) return 0;
     return sp;
 }
 // Variables per workgroup of a sweep kind (viscous flux sweeps run one
@@ -1473,6 +1483,7 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
                 B.part[d][e] = s.partner[d][e];
                 const int cd = s.bc[d][e];
                 B.qd[d][e] = (cd == kInterStraight || cd == kInterSub || cd == kPeriodic) ? 1 : 0;
+                if (B.qd[d][e] && B.part[d][e] != b) c->part_other[d] = true;
                 if (cd == kNonReflective || cd == kWallInviscid || cd == kWallViscous || cd == kInterCurv) {
                     c->any_gcbc = true; B.gcbc_any = 1;
                 }

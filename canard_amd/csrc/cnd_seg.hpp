@@ -84,13 +84,17 @@ struct SegShape {
     __host__ __device__ static constexpr int ends_doubles() { return 2 * kSegEndRows * NV * P; }
     // (xi: the whole tile's sources [nmax][RS] replace the work area; once
     // the closure rows are formed, the LU rows and then the exchange buffer
-    // [4][NV][SP][P] overlay the tile.)
-    __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax)
+    // [4][NV][SP][P] overlay the tile; the partner points [2][13][NV][P]
+    // come last and only when has_part, some block having another block as
+    // its partner along the sweep direction.)
+    __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax, bool has_part)
     {
         const size_t tile = size_t(RS) * nmax, ex4 = size_t(4) * NV * SP * P, lu = size_t(5) * L * spm;
         const size_t over = lu + ex4;
-        const size_t work = XI ? (tile > over ? tile : over) : size_t(work_doubles(SP)) + lu;
-        return sizeof(double) * (work + size_t(ends_doubles()) + size_t(8) * spm);
+        if (XI)
+            return sizeof(double) * ((tile > over ? tile : over) + size_t(8) * spm +
+                                     (has_part ? size_t(ends_doubles()) : size_t(0)));
+        return sizeof(double) * (size_t(work_doubles(SP)) + lu + size_t(ends_doubles()) + size_t(8) * spm);
     }
 };
 
@@ -255,30 +259,28 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     // exchange buffer), the pencil-end values, SM, LU rows.  xi: the whole
     // tile's sources [n][NV][P] (later the exchange buffer and the results),
     // the partner points, SM, LU rows.
-    // the LU rows overlay the xi tile (written after the closure rows)
+    // xi: the LU rows overlay the tile (written after the closure rows)
     constexpr bool LU_OVER = XI;
     const int nlu = 5 * L * SP;
     double *buf = lds;
     double *ends = nullptr, *part = nullptr, *LUs = nullptr, *ex = nullptr;
+    double *SM = nullptr;   // [SPM][8]
     if constexpr (XI) {
+        // [tile, later LU rows + exchange buffer][SM][partner points, only
+        // allocated when some block has another block as partner along d]
         const int fsz = Sh::RS * n, exs4 = 4 * NV * SP * P;
-        if constexpr (LU_OVER) {
-            LUs = lds;
-            ex = lds + nlu;
-            part = lds + (fsz > nlu + exs4 ? fsz : nlu + exs4);
-        } else {
-            ex = lds + fsz;
-            LUs = ex + exs4;
-            part = LUs + 5 * L * SPM;
-        }
+        LUs = lds;
+        ex = lds + nlu;
+        SM = lds + (fsz > nlu + exs4 ? fsz : nlu + exs4);
+        part = SM + 8 * SPM;
         ends = part;
     } else {
         ends = lds + Sh::work_doubles(SP);
         part = buf;
         ex = buf;
-        LUs = ends + Sh::ends_doubles() + 8 * SPM;   // after SM
+        SM = ends + Sh::ends_doubles();
+        LUs = SM + 8 * SPM;
     }
-    double *SM = XI ? part + Sh::ends_doubles() : ends + Sh::ends_doubles();   // [SPM][8]
     constexpr int RS = Sh::RS;                                              // xi: staged row stride
     auto F = [&](int row) { return buf[row * RS + v * P + p]; };            // xi: staged source
     const bool pen_ok = tile.q0 + p < npen;
@@ -453,40 +455,58 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     // values within about 24 doubles of registers
     constexpr int RAWD = int(sizeof(typename Src::Raw) / sizeof(double));
     constexpr int BAT = (24 / RAWD) < 1 ? 1 : ((24 / RAWD) < CPT ? 24 / RAWD : CPT);
+    // the loads of slot k of chunk c (addresses clamped into the block, so
+    // they are valid for every slot) and the LDS place of its sources
+    auto issue = [&](int c, int k, typename Src::Raw &raw, int &dst) {
+        const int slot = k < CPT ? t + k * nthr : nslots;
+        int o, ss, pp;
+        seg_slot<P, C, XI>(slot < nslots ? slot : 0, SP, o, ss, pp);
+        const int ssc = ss < S ? ss : S - 1;
+        int sa, sl;
+        seg_bounds(n, L, S, ssc, sa, sl);
+        const int row = sa - (L - sl) - H + c * C + o;
+        const bool ok = slot < nslots && ss < S && tile.q0 + pp < npen && row >= 0 && row <= N &&
+                        row >= sa - H && row < sa + sl + H;
+        const int rowc = row < 0 ? 0 : (row > N ? N : row);
+        const int ppc = tile.q0 + pp < npen ? pp : npen - 1 - tile.q0;
+        src.load(pt(ppc, rowc), b, raw);
+        dst = ok ? ((o * NV) * SP + ss) * P + pp : -1;
+    };
+    auto deposit = [&](const typename Src::Raw &raw, int dst) {
+        if (dst < 0) return;
+        double f[NV];
+        src.finish(raw, b, f);
+#pragma unroll
+        for (int m = 0; m < NV; ++m) buf[dst + m * SP * P] = f[m];
+    };
+    // one batch per chunk: the next chunk's loads are issued before this
+    // chunk's rows are consumed, so they are in flight meanwhile
+    typename Src::Raw rawp[BAT];
+    int dstp[BAT];
+    if constexpr (BAT == CPT) {
+#pragma unroll
+        for (int kk = 0; kk < CPT; ++kk) issue(0, kk, rawp[kk], dstp[kk]);
+    }
 #pragma unroll
     for (int c = 0; c < Sh::NCH; ++c) {
         __syncthreads();
+        if constexpr (BAT == CPT) {
 #pragma unroll
-        for (int k0 = 0; k0 < CPT; k0 += BAT) {
-        // a group's loads first (addresses clamped into the block, so they are
-        // valid for every slot), then its sources into LDS
-        typename Src::Raw raw[BAT];
-        int dst[BAT];
+            for (int kk = 0; kk < CPT; ++kk) deposit(rawp[kk], dstp[kk]);
+            if (c + 1 < Sh::NCH) {
 #pragma unroll
-        for (int kk = 0; kk < BAT; ++kk) {
-            const int k = k0 + kk;
-            const int slot = k < CPT ? t + k * nthr : nslots;
-            int o, ss, pp;
-            seg_slot<P, C, XI>(slot < nslots ? slot : 0, SP, o, ss, pp);
-            const int ssc = ss < S ? ss : S - 1;
-            int sa, sl;
-            seg_bounds(n, L, S, ssc, sa, sl);
-            const int row = sa - (L - sl) - H + c * C + o;
-            const bool ok = slot < nslots && ss < S && tile.q0 + pp < npen && row >= 0 && row <= N &&
-                            row >= sa - H && row < sa + sl + H;
-            const int rowc = row < 0 ? 0 : (row > N ? N : row);
-            const int ppc = tile.q0 + pp < npen ? pp : npen - 1 - tile.q0;
-            src.load(pt(ppc, rowc), b, raw[kk]);
-            dst[kk] = ok ? ((o * NV) * SP + ss) * P + pp : -1;
-        }
+                for (int kk = 0; kk < CPT; ++kk) issue(c + 1, kk, rawp[kk], dstp[kk]);
+            }
+        } else {
 #pragma unroll
-        for (int kk = 0; kk < BAT; ++kk) {
-            if (dst[kk] < 0) continue;
-            double f[NV];
-            src.finish(raw[kk], b, f);
+            for (int k0 = 0; k0 < CPT; k0 += BAT) {
+                typename Src::Raw raw[BAT];
+                int dst[BAT];
 #pragma unroll
-            for (int m = 0; m < NV; ++m) buf[dst[kk] + m * SP * P] = f[m];
-        }
+                for (int kk = 0; kk < BAT; ++kk) issue(c, k0 + kk, raw[kk], dst[kk]);
+#pragma unroll
+                for (int kk = 0; kk < BAT; ++kk) deposit(raw[kk], dst[kk]);
+            }
         }
         __syncthreads();
 #pragma unroll
