@@ -147,6 +147,8 @@ struct SinkOut {   // out fields vs apart, stored as TO
         for (int k = 0; k < NV; ++k) o[k * vs + g] = TO(x[k]);
     }
     __device__ void put(int64_t g, int, int k, double x) { o[k * vs + g] = TO(x); }
+    static constexpr bool kReadBack = false;
+    __device__ double src(int64_t, int) const { return 0.0; }
     __device__ double fold(double, double x) const { return x; }
     __device__ void put_raw(int64_t g, int b, int k, double x) { put(g, b, k, x); }
     __device__ void ends(int, int, int, double) {}
@@ -163,6 +165,8 @@ struct SinkFilt {  // out = f + correction
         for (int k = 0; k < NV; ++k) o[k * vs + g] = f[k * vs + g] + x[k];
     }
     __device__ void put(int64_t g, int, int k, double x) { o[k * vs + g] = f[k * vs + g] + x; }
+    static constexpr bool kReadBack = true;   // results are added to the sources
+    __device__ double src(int64_t g, int k) const { return f[k * vs + g]; }
     __device__ double fold(double fv, double x) const { return fv + x; }
     __device__ void put_raw(int64_t g, int, int k, double x) { o[k * vs + g] = x; }
     __device__ void ends(int, int, int, double) {}
@@ -177,6 +181,8 @@ struct SinkFlux {  // the five components of dF/dxi_d; pencil ends to drva
         for (int m = 0; m < 5; ++m) de[m * NT + g] = x[m];
     }
     __device__ void put(int64_t g, int, int m, double x) { de[m * NT + g] = x; }
+    static constexpr bool kReadBack = false;
+    __device__ double src(int64_t, int) const { return 0.0; }
     __device__ double fold(double, double x) const { return x; }
     __device__ void put_raw(int64_t g, int b, int m, double x) { put(g, b, m, x); }
     __device__ void ends(int side, int lane, int m, double x) { drva[(side * 5 + m) * FT + fi0 + lane] = x; }
@@ -188,6 +194,8 @@ struct SinkFluxV {  // component v of dF/dxi_d; pencil ends to drva
     int64_t FT, fi0;
     __device__ void store(int64_t g, int, const double (&x)[1]) { o[g] = x[0]; }
     __device__ void put(int64_t g, int, int, double x) { o[g] = x; }
+    static constexpr bool kReadBack = false;
+    __device__ double src(int64_t, int) const { return 0.0; }
     __device__ double fold(double, double x) const { return x; }
     __device__ void put_raw(int64_t g, int b, int k, double x) { put(g, b, k, x); }
     __device__ void ends(int side, int lane, int, double x) { drva[(side * 5 + v) * FT + fi0 + lane] = x; }

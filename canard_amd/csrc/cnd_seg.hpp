@@ -43,6 +43,9 @@ constexpr int kSegMinRows = 12;
 // LU values a thread stages per round trip.
 constexpr int kLuBatch = 12;
 
+// Points whose sources a thread reads back together at a xi store.
+constexpr int kOutBatch = 8;
+
 // Segments of an n-point pencil for lanes of at most L points: 0 when no
 // partition fits (a segment holding a pencil end must contain the 12 points
 // its closure rows use).  Segments are as even as possible.
@@ -778,13 +781,39 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         }
         __syncthreads();
         const int tot = P * n;
-        for (int e = t; e < tot; e += nthr) {
-            const int pp = e / n, row = e % n;
-            if (tile.q0 + pp >= npen) continue;
-            double x[NV];
+        if constexpr (Sink::kReadBack) {
+            // the sink adds the results to its sources: read those back for
+            // kOutBatch points at a time, all loads in flight together
+            for (int e0 = t; e0 < tot; e0 += kOutBatch * nthr) {
+                double fs[kOutBatch][NV];
+                int64_t gg[kOutBatch];
 #pragma unroll
-            for (int m = 0; m < NV; ++m) x[m] = buf[row * RS + m * P + pp];
-            sink.store(pt(pp, row), b, x);
+                for (int kk = 0; kk < kOutBatch; ++kk) {
+                    const int e = e0 + kk * nthr < tot ? e0 + kk * nthr : tot - 1;
+                    const int pp = e / n, row = e % n;
+                    gg[kk] = pt(tile.q0 + pp < npen ? pp : npen - 1 - tile.q0, row);
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) fs[kk][m] = sink.src(gg[kk], m);
+                }
+#pragma unroll
+                for (int kk = 0; kk < kOutBatch; ++kk) {
+                    const int e = e0 + kk * nthr;
+                    if (e >= tot || tile.q0 + e / n >= npen) continue;
+                    const int row = e % n, pp = e / n;
+#pragma unroll
+                    for (int m = 0; m < NV; ++m)
+                        sink.put_raw(gg[kk], b, m, sink.fold(fs[kk][m], buf[row * RS + m * P + pp]));
+                }
+            }
+        } else {
+            for (int e = t; e < tot; e += nthr) {
+                const int pp = e / n, row = e % n;
+                if (tile.q0 + pp >= npen) continue;
+                double x[NV];
+#pragma unroll
+                for (int m = 0; m < NV; ++m) x[m] = buf[row * RS + m * P + pp];
+                sink.store(pt(pp, row), b, x);
+            }
         }
     }
 
