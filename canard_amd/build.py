@@ -42,23 +42,34 @@ def hipcc():
 
 
 def build_lib(verbose=False, lib=LIB, defines=()):
-    """Compile the HIP sources into lib; defines (e.g. "CND_CHUNK_ETA=16")
-    override compile-time tile constants for tuning builds."""
-    objs = []
+    """Compile the HIP sources (in parallel) into lib; defines (e.g.
+    "CND_CHUNK_ETA=16") override compile-time tile constants for tuning
+    builds."""
     objdir = os.path.join(HERE, "_obj")
     if lib != LIB:
         objdir = os.path.join(objdir, os.path.basename(lib).replace(".", "_"))
     os.makedirs(objdir, exist_ok=True)
+    jobs = []
     for s in SOURCES:
         src = os.path.join(CSRC, s)
         if not os.path.exists(src):
             continue
         obj = os.path.join(objdir, s + ".o")
         lang = ["-x", "hip"] if s.endswith(".hip") else []
-        out = _run([hipcc()] + HIPFLAGS + ["-D" + d for d in defines] + lang + ["-c", src, "-o", obj])
-        if verbose and out.strip():
+        cmd = [hipcc()] + HIPFLAGS + ["-D" + d for d in defines] + lang + ["-c", src, "-o", obj]
+        jobs.append((cmd, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    objs = []
+    failed = None
+    for cmd, obj, proc in jobs:
+        out, _ = proc.communicate()
+        if proc.returncode != 0:
+            sys.stderr.write(out)
+            failed = failed or cmd
+        elif verbose and out.strip():
             print(out)
         objs.append(obj)
+    if failed:
+        raise RuntimeError("build step failed: " + " ".join(failed))
     tmp = lib + ".tmp"
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, lib)
