@@ -13,6 +13,9 @@
 // solutions phi1, phi2 / psi1, psi2 at the segment's ends) turns them into
 // the true incoming values; the second run starts from those and is the
 // sequential recurrence itself, so the result equals it up to rounding.
+// Segments are held right-aligned in L registers; the registers in front of
+// a shorter segment are pass-through rows (LU coefficients that swap the
+// recurrence state exactly), so every lane runs the same branch-free code.
 //
 // Workgroup: NV * P * SP threads.  Thread t: variable v = t / (P * SP),
 // pencil p = t % P, segment s = (t / P) % SP.  Sources reach the lanes in
@@ -23,9 +26,9 @@
 // eta/zeta sweeps of derived sources (primitives, inviscid fluxes) evaluate
 // them cooperatively chunk by chunk (C rows of every segment) into LDS and
 // each lane picks its variable.  The lanes holding a pencil end keep the 13
-// source values next to it in LDS; the closure rows are formed from them
-// after the stream (and, for a block that is its own periodic partner, the
-// partner sums too, so no extra loads).
+// source values next to it in LDS (xi: they stay in the staged tile); the
+// closure rows are formed from them after the stream (and, for a block that
+// is its own periodic partner, the partner sums too, so no extra loads).
 #pragma once
 
 #include "cnd_dev.hpp"
