@@ -1,7 +1,7 @@
 """Mixed precision (precision=CND_MIXED): FP32 line solves with one FP64
 residual refinement in every sweep, FP32 storage of the velocity/temperature
-gradients and the viscous stresses; state, metrics, RK accumulation and GCBC
-stay FP64.  Pinned by the FP64 device path and the oracle.
+gradients; stresses, fluxes, state, metrics, RK accumulation and GCBC stay
+FP64.  Pinned by the FP64 device path and the oracle.
 
 Tolerances: a refined sweep is within 1e-12 of the field maximum of the FP64
 result (one refinement of an FP32 solve of a well-conditioned banded system
@@ -49,10 +49,12 @@ def test_mixed_sweeps_match_oracle(name):
     o.close()
 
 
-def test_mixed_steps_within_target_of_fp64():
+@pytest.mark.parametrize("viscous", [1, 0])
+def test_mixed_steps_within_target_of_fp64(viscous):
     from canard_amd.solver import CND_FP64, CND_MIXED
     blocks, grids, states, params = tgv_case(48, pert=1e-3)
-    print("[mixed steps] tgv 48^3, 20 steps", flush=True)
+    params = dict(params, viscous=viscous)
+    print(f"[mixed steps] tgv 48^3, 20 steps, viscous={viscous}", flush=True)
     a = _case(blocks, grids, states, params, CND_FP64)
     b = _case(blocks, grids, states, params, CND_MIXED)
     a.step(20)

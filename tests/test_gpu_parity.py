@@ -162,6 +162,43 @@ def test_multiblock_gcbc_sponge_matches_oracle(sweep_mode):
     o.close()
 
 
+def test_inviscid_steps_match_oracle(sweep_mode):
+    """Euler steps (viscous = 0): the fluxes are evaluated inside the flux
+    sweeps instead of being assembled with the stresses.  A TGV block and
+    the two-block GCBC + sponge case; tolerances as for the viscous cases."""
+    blocks, grids, states, params = tgv_case(24, pert=1e-3)
+    params = dict(params, viscous=0)
+    print("[steps] inviscid tgv 24", flush=True)
+    c = _case(blocks, grids, states, params)
+    o = oracle_for(blocks, grids, states, params)
+    c.step(6)
+    o.run(6)
+    err = rel_err(c.state(0), o.state(0))
+    print("  rel err per variable", err)
+    assert np.all(err < 1e-12), err
+    c.close()
+    o.close()
+    sizes = [(24, 16, 14), (16, 16, 14)]
+    blocks = cases.sheared_pair_blocks(sizes)
+    grids = cases.sheared_pair_grids(sizes, h=1.0, amp=0.05)
+    states = [cases.pulse_state(g, mach=0.5, centres=[(20.0, 8.0, 7.0)], amp=1e-3, width=3.0) for g in grids]
+    params = dict(reoo=200.0, amach1=0.5, nsmf=1, cfl=0.8, sponge=1, szco=0.5, szth0=6.0, szth1=6.0,
+                  doml0=0.0, doml1=40.0, viscous=0)
+    print("[steps] inviscid multiblock gcbc sponge", flush=True)
+    c = _case(blocks, grids, states, params)
+    o = oracle_for(blocks, grids, states, params)
+    c.step(6)
+    o.run(6)
+    tol_change = 1e-11 if sweep_mode == "tiled" else 1e-10
+    for b in range(2):
+        err = rel_err(c.state(b) - states[b], o.state(b) - states[b])
+        print("  block", b, "rel err of the change", err)
+        assert np.all(err < tol_change), err
+        assert np.all(rel_err(c.state(b), o.state(b)) < 1e-12)
+    c.close()
+    o.close()
+
+
 def test_ensemble_instances_match_separate_oracles():
     """Three independent instances of different size, Mach and Re in one
     context; each must equal its own oracle run."""
