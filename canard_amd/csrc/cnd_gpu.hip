@@ -332,8 +332,8 @@ using Sh16 = SegG<17, 24, 16, 16, true>;
 // 1024-thread limit are rejected at launch time by shape_sp)
 __host__ __device__ constexpr int wg_bound(int x) { return x > 1024 ? 1024 : x; }
 // waves per SIMD a segmented sweep is compiled for: xi shapes of 17-point
-// segments are held to 128 VGPRs (4 waves), where their LDS also allows four
-// workgroups of 16 pencils per CU; the others are left to the compiler
+// segments are held to 128 VGPRs (4 waves), which their LDS also allows;
+// the others are left to the compiler
 template <class G>
 __host__ __device__ constexpr int seg_waves() { return G::XI && G::L <= 17 ? 4 : 1; }
 constexpr int kTiled = -1;   // shape index of the tiled sweep
@@ -1589,14 +1589,14 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
                 ok(dalloc(c, &c->dlu, lu.size())) && ok(dalloc(c, &c->qa, 5 * NT)) && ok(dalloc(c, &c->qo, 5 * NT)) &&
                 ok(dalloc(c, &c->de, 15 * NT)) && ok(dalloc(c, &c->grad, 12 * NT)) && ok(dalloc(c, &c->fl, 15 * NT)) &&
                 ok(dalloc(c, &c->met, 10 * NT)) && ok(dalloc(c, &c->ys, c->ys_len)) && ok(dalloc(c, &c->xyz, 3 * NT)) &&
-                ok(dalloc(c, &c->sp_par, 5 * ninst)) && ok(dalloc(c, &c->dseg[0], segtab[0].size())) &&
-                ok(dalloc(c, &c->dseg[1], segtab[1].size()));
+                ok(dalloc(c, &c->sp_par, 5 * ninst));
+    for (int li = 0; good && li < kNumL; ++li) good = ok(dalloc(c, &c->dseg[li], segtab[li].size()));
     if (good && c->any_sponge) good = ok(dalloc(c, &c->sponge, 2 * NT));
     if (good && precision == CND_MIXED)
         good = ok(dalloc(c, &c->gradf, 12 * NT)) &&
-               ok(dalloc(c, &c->dlu32, lu.size())) && ok(dalloc(c, &c->dseg32[0], segtab[0].size())) &&
-               ok(dalloc(c, &c->dseg32[1], segtab[1].size())) &&
-               ok(dalloc(c, &c->dlhs, lhs.size()));
+               ok(dalloc(c, &c->dlu32, lu.size())) && ok(dalloc(c, &c->dlhs, lhs.size()));
+    for (int li = 0; good && precision == CND_MIXED && li < kNumL; ++li)
+        good = ok(dalloc(c, &c->dseg32[li], segtab[li].size()));
     for (int d = 0; good && d < 3; ++d)
         good = ok(dalloc(c, &c->dtiles[d], c->htiles[d].size())) && ok(dalloc(c, &c->drva[d], 10 * c->FT[d])) &&
                ok(dalloc(c, &c->cmf[d], 6 * c->FT[d])) && ok(dalloc(c, &c->surf[d], 10 * c->FT[d]));
@@ -1684,9 +1684,13 @@ void cnd_destroy(cnd_ctx *c)
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->fl, c->met, c->ys,
-                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->dlu32, c->dseg32[0], c->dseg32[1], c->dlhs, c->jc.tgt, c->jc.src, c->jc.first, c->jc.scale,
-                    c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp, c->dseg[0], c->dseg[1]};
+                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->dlu32, c->dlhs, c->jc.tgt, c->jc.src,
+                    c->jc.first, c->jc.scale, c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp};
     for (void *p : ptrs) if (p) hipFree(p);
+    for (int li = 0; li < kNumL; ++li) {
+        if (c->dseg[li]) hipFree(c->dseg[li]);
+        if (c->dseg32[li]) hipFree(c->dseg32[li]);
+    }
     for (int d = 0; d < 3; ++d) {
         if (c->dtiles[d]) hipFree(c->dtiles[d]);
         for (int pi = 0; pi < kNumP; ++pi)
