@@ -263,17 +263,14 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     const double *ST = tb.seg + 4 * int64_t(B.lu[SCH][d]);
     // LDS.  eta/zeta: work area (chunk buffer, then partner points, then the
     // exchange buffer), the pencil-end values, SM, LU rows.  xi: the whole
-    // tile's sources [n][NV][P] (later the exchange buffer and the results),
-    // the partner points, SM, LU rows.
-    // xi: the LU rows overlay the tile (written after the closure rows)
-    constexpr bool LU_OVER = XI;
+    // tile's sources [n][RS] (once the closure rows are formed: the LU rows
+    // and the exchange buffer; at the end: the results), SM, and the
+    // partner points where some block has another block as partner along d.
     const int nlu = 5 * L * SP;
     double *buf = lds;
     double *ends = nullptr, *part = nullptr, *LUs = nullptr, *ex = nullptr;
     double *SM = nullptr;   // [SPM][8]
     if constexpr (XI) {
-        // [tile, later LU rows + exchange buffer][SM][partner points, only
-        // allocated when some block has another block as partner along d]
         const int fsz = Sh::RS * n, exs4 = 4 * NV * SP * P;
         LUs = lds;
         ex = lds + nlu;
@@ -315,7 +312,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         const double x = LU[real ? 5 * (sa - (L - sl) + k) + c : 0];
         return real ? x : seg_pass_row(c);
     };
-    // LU_OVER: the first KPRE values per thread are loaded once the tile is
+    // xi: the first KPRE values per thread are loaded once the tile is
     // staged (their latency overlaps the right-hand sides and closure rows)
     // and written over the tile afterwards
     constexpr int KPRE0 = (5 * L + NV * P - 1) / (NV * P);
@@ -338,7 +335,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
                 if (i0 + u * nthr < nlu) LUs[i0 + u * nthr] = w2[u];
         }
     };
-    if constexpr (!LU_OVER) {
+    if constexpr (!XI) {
         lu_fetch();
         lu_store();
     }
@@ -392,7 +389,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             }
         }
         __syncthreads();
-        if constexpr (LU_OVER) lu_fetch();
+        lu_fetch();
 #pragma unroll
         for (int kr = 0; kr < Sh::KR; ++kr) {
             const int row = a - kb - H + kr;
@@ -681,7 +678,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         for (int r = 0; r < Sh::NB; ++r) Y[L - 1 - r] = hi[r];
     }
 
-    if constexpr (LU_OVER) {   // the tile is no longer read: LU rows over it
+    if constexpr (XI) {   // the tile is no longer read: LU rows over it
         __syncthreads();
         lu_store();
         __syncthreads();
