@@ -1,0 +1,26 @@
+"""bench.py's command line on the CPU: more than one GPU is refused with a
+message (one GPU per run in this build), and the options parse."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench(*args):
+    return subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], capture_output=True, text=True,
+                          timeout=120)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = _bench("--gpus", "2")
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
+    assert r.stdout.strip() == ""
+
+
+def test_help_lists_the_contract_options():
+    r = _bench("--help")
+    assert r.returncode == 0
+    for opt in ("--gpus", "--steps", "--warmup", "--shapes", "--precision", "--cpu-steps"):
+        assert opt in r.stdout
