@@ -33,6 +33,10 @@ namespace {
 
 constexpr int kLine = kTile;  // pencils per face/sweep workgroup (one wave)
 constexpr int kPts = 256;     // points per pointwise workgroup
+// Every block's fields start on a multiple of this many points (2 KiB), so
+// that workgroups reading or writing whole tiles of points touch whole
+// cache lines of every field; the padding points are never read.
+constexpr int64_t kAlignPts = 256;
 
 // ---- sources (values a sweep differentiates or filters) ---------------------
 template <int NV>
@@ -1019,6 +1023,7 @@ struct cnd_ctx {
     std::vector<int64_t> off;
     int64_t NT = 0, FT[3] = {0, 0, 0};
     long long cells = 0;
+    int64_t npts = 0;   // stored points (NT, the field stride, also counts the alignment padding)
     hipStream_t st = nullptr;
     BlkDev *dblk = nullptr;
     TabDev *dtab = nullptr;
@@ -1503,7 +1508,8 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         B.off = off;
         c->off.push_back(off);
         const int64_t npt = int64_t(s.n[0]) * s.n[1] * s.n[2];
-        off += npt;
+        c->npts += npt;
+        off += (npt + kAlignPts - 1) / kAlignPts * kAlignPts;   // next block's fields start aligned
         c->cells += (long long)(s.n[0] - 1) * (s.n[1] - 1) * (s.n[2] - 1);
         for (int d = 0; d < 3; ++d) {
             B.foff[d] = foff[d];
@@ -2076,7 +2082,7 @@ int cnd_profile_read(cnd_ctx *c, int section, double *ms, long long *launches)
 }
 
 void *cnd_stream(cnd_ctx *c) { return c ? (void *)c->st : nullptr; }
-long long cnd_points(const cnd_ctx *c) { return c ? (long long)c->NT : 0; }
+long long cnd_points(const cnd_ctx *c) { return c ? (long long)c->npts : 0; }
 long long cnd_cells(const cnd_ctx *c) { return c ? c->cells : 0; }
 
 }  // extern "C"
