@@ -205,6 +205,39 @@ struct SinkFluxV {  // component v of dF/dxi_d; pencil ends to drva
     __device__ void ends(int side, int lane, int, double x) { drva[(side * 5 + v) * FT + fi0 + lane] = x; }
 };
 
+// Component v of dF/dzeta folded straight into the RK update of variable v
+// (k_update's arithmetic without the sponge, same operation order, so results
+// are bit-identical): qa = qo - dt/(5-nk) * yaco * ((de_xi + de_eta) + x).
+// Every operand is variable v at the sink's own point.  Pre holds one point's
+// operands, so sweeps can load them for several points before storing any.
+struct SinkUpdateV {
+    double *qa, *drva;
+    const double *qo, *de0, *de1, *met;
+    double dtk;   // dt / (5 - nk) of the tile's instance
+    int v;
+    int64_t NT, FT, fi0;
+    struct Pre { double s, q, r; };
+    __device__ void pre(int64_t g, Pre &o) const
+    {
+        o.s = de0[g] + de1[g];
+        o.q = qo[g];
+        o.r = dtk * met[9 * NT + g];
+    }
+    __device__ void put_pre(int64_t g, double x, const Pre &o) { qa[g] = o.q - o.r * (o.s + x); }
+    __device__ void put(int64_t g, int, int, double x)
+    {
+        Pre o;
+        pre(g, o);
+        put_pre(g, x, o);
+    }
+    __device__ void store(int64_t g, int b, const double (&x)[1]) { put(g, b, 0, x[0]); }
+    static constexpr bool kReadBack = false;
+    __device__ double src(int64_t, int) const { return 0.0; }
+    __device__ double fold(double, double x) const { return x; }
+    __device__ void put_raw(int64_t g, int b, int k, double x) { put(g, b, k, x); }
+    __device__ void ends(int side, int lane, int, double x) { drva[(side * 5 + v) * FT + fi0 + lane] = x; }
+};
+
 // ---- sweep kernels -----------------------------------------------------------
 // grid: (xcd_width(tiles of direction d), variable groups); NV waves per tile
 // of 64 pencils; C = chunk_of(d); dynamic LDS = sweep_lds_bytes<NV, C>().
@@ -417,6 +450,25 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
     const Tile t = tiles[tix];
     SrcField<1> src{F + v * NT, 0};
     SinkFluxV sink{de + v * NT, drva, v, FT, blks[t.b].foff[d] + t.q0};
+    seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
+}
+
+// Last flux sweep of a stage fused with the RK update: sweep of the stored
+// flux component v (grid y) along d, written as the updated state qa.
+template <class G, bool MIX>
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_update(
+    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
+    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *F, const double *de,
+    const double *__restrict__ qo, const double *__restrict__ met, double *qa, double *drva, int64_t NT, int64_t FT,
+    int nk)
+{
+    SEG_PROLOGUE(ntiles)
+    const int v = blockIdx.y;
+    const Tile t = tiles[tix];
+    SrcField<1> src{F + v * NT, 0};
+    const double dtk = inst[blks[t.b].inst].dt / double(4 - nk + 1);
+    SinkUpdateV sink{qa + v * NT, drva, qo + v * NT, de + v * NT, de + (5 + v) * NT, met, dtk, v, NT, FT,
+                     blks[t.b].foff[d] + t.q0};
     seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
 }
 
@@ -1041,6 +1093,7 @@ struct cnd_ctx {
     int shape[CND_NKIND][3] = {};
     int seg_sp[CND_NKIND][3] = {};
     bool tuned = false, tune_on = true, tiled_only = false;
+    bool fuse_update = true;   // last viscous flux sweep writes the RK update (CANARD_FUSE_UPDATE=0: separate)
     int lds_max = 65536;   // dynamic LDS per workgroup the device allows
     double tune_ms[CND_NKIND][3] = {};
     std::vector<Tile> hpt;
@@ -1322,6 +1375,21 @@ int run_flux(cnd_ctx *c, int d, const double *cur)
     return check_launch(c);
 }
 
+// zeta flux sweep of the stored fluxes fused with the RK update into qa
+// (viscous stages without sponge, segmented zeta shape or mixed precision).
+int run_flux_update(cnd_ctx *c, int nk)
+{
+    const int K = CND_KIND_FLUX, d = 2;
+    const double *F = c->fl + 5 * d * c->NT;
+    if (c->precision == CND_MIXED)
+        LAUNCH_SEG(k_seg_flux_update, true, 1, c, K, d, 5, c->dinst, F, c->de, c->qo, c->met, c->qa, c->drva[d],
+                   c->NT, c->FT[d], nk);
+    else
+        LAUNCH_SEG(k_seg_flux_update, false, 1, c, K, d, 5, c->dinst, F, c->de, c->qo, c->met, c->qa, c->drva[d],
+                   c->NT, c->FT[d], nk);
+    return check_launch(c);
+}
+
 // Filter phase (mo_canard_driver.f90:233-252): eta, zeta, xi sweeps of all
 // five variables, out of place so partner reads see the pre-sweep field; the
 // filtered state (which is also qo) ends in qo.
@@ -1364,9 +1432,14 @@ int enqueue_stage(cnd_ctx *c, int nk)
             hipLaunchKernelGGL(k_visc_flux<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt,
                                cur, c->grad, c->met, c->fl, c->NT);
     }
+    // Viscous stages without characteristic faces or sponge fold the update
+    // into the zeta flux sweep (segmented shapes only); GCBC needs all three
+    // flux derivatives before the update.
+    const bool fused = c->fuse_update && c->any_visc && !c->any_gcbc && !c->any_sponge &&
+                       (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, 2));
     {
-    Section sec(c, CND_SEC_FLUX, 3);
-    for (int d = 0; d < 3; ++d)
+    Section sec(c, CND_SEC_FLUX, fused ? 2 : 3);
+    for (int d = 0; d < (fused ? 2 : 3); ++d)
         if ((r = run_flux(c, d, cur))) return r;
     }
     if (c->any_gcbc) {
@@ -1380,7 +1453,10 @@ int enqueue_stage(cnd_ctx *c, int nk)
                                    c->dtab, c->dtiles[d], d, s, cur, c->de, c->met, c->cmf[d], c->drva[d], c->NT,
                                    c->FT[d]);
     }
-    {
+    if (fused) {
+        Section sec(c, CND_SEC_UPDATE, 1);
+        if ((r = run_flux_update(c, nk))) return r;
+    } else {
         Section sec(c, CND_SEC_UPDATE, 1);
         hipLaunchKernelGGL(k_update, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->qo, c->de,
                            c->met, c->sponge, c->qa, c->NT, nk);
@@ -1552,6 +1628,8 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         c->tiled_only = sw && std::string(sw) == "tiled";
         const char *tu = std::getenv("CANARD_TUNE");
         c->tune_on = !c->tiled_only && !(tu && std::string(tu) == "0");
+        const char *fu = std::getenv("CANARD_FUSE_UPDATE");
+        c->fuse_update = !(fu && std::string(fu) == "0");
         {
             int dev = 0, lds = 0;
             if (hipGetDevice(&dev) == hipSuccess &&

@@ -31,6 +31,8 @@
 // is its own periodic partner, the partner sums too, so no extra loads).
 #pragma once
 
+#include <type_traits>
+
 #include "cnd_dev.hpp"
 #include "cnd_kernels.hpp"
 
@@ -235,12 +237,19 @@ struct SegTabs {
     const float *lu32, *seg32;
 };
 
+// Sinks that read operands of their own at each point (type Pre, filled by
+// pre(g, Pre&), written with put_pre(g, x, Pre)): eta/zeta sweeps load them
+// for kOutBatch points before storing any.
+template <class S, class = void> struct sink_pre : std::false_type {};
+template <class S> struct sink_pre<S, std::void_t<typename S::Pre>> : std::true_type {};
+
 // One tile of P pencils of block tile.b along d.
 //  Src: load(g, b, Raw&), finish(Raw, b, double (&)[NV])
 //  Sink: store(g, b, const double (&)[NV]), put(g, b, v, x), ends(side, pencil, v, x);
 //        fold(f, x): the value to write given the source f and the result x,
 //        put_raw: write such a value (eta/zeta sweeps of plain fields, which
-//        still hold the source in registers, use these instead of put)
+//        still hold the source in registers, use these instead of put);
+//        optionally Pre, pre and put_pre (sink_pre)
 // MIX: solve in FP32 and refine once with the FP64 residual r = b - A x
 // (A the banded LHS), x += A^-1 r solved in FP32.
 template <int NV, int SCH, int L, int P, int C, bool XI, bool MIX, int SPM, class Src, class Sink>
@@ -759,14 +768,28 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
             int64_t pb, st;
             pencil_of(B, d, tile.q0 + p, pb, st);
             pb += int64_t(a - kb) * st;
+            if constexpr (sink_pre<Sink>::value) {
+                static_assert(!Sink::kReadBack, "prefetching sinks write results as they are");
 #pragma unroll
-            for (int k = 0; k < L; ++k)
-                if (have(k)) {
-                    if constexpr (Src::kDirect)
-                        sink.put_raw(pb + int64_t(k) * st, b, v, sink.fold(W[k + H], Y[k]));
-                    else
-                        sink.put(pb + int64_t(k) * st, b, v, Y[k]);
+                for (int k0 = 0; k0 < L; k0 += kOutBatch) {
+                    typename Sink::Pre pr[kOutBatch];
+#pragma unroll
+                    for (int kk = 0; kk < kOutBatch; ++kk)
+                        if (k0 + kk < L && have(k0 + kk)) sink.pre(pb + int64_t(k0 + kk) * st, pr[kk]);
+#pragma unroll
+                    for (int kk = 0; kk < kOutBatch; ++kk)
+                        if (k0 + kk < L && have(k0 + kk)) sink.put_pre(pb + int64_t(k0 + kk) * st, Y[k0 + kk], pr[kk]);
                 }
+            } else {
+#pragma unroll
+                for (int k = 0; k < L; ++k)
+                    if (have(k)) {
+                        if constexpr (Src::kDirect)
+                            sink.put_raw(pb + int64_t(k) * st, b, v, sink.fold(W[k + H], Y[k]));
+                        else
+                            sink.put(pb + int64_t(k) * st, b, v, Y[k]);
+                    }
+            }
         }
     } else {
         // xi: results into the tile buffer, then stored along the pencils
