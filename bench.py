@@ -32,7 +32,7 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s (AMD public figure)
 STEP_BYTES_MODEL = 5048.0  # compulsory FP64 bytes per stored point per viscous step
 
 # Algorithmic HBM bytes per stored point of one launch of each kernel group
-# (reads + writes of FP64 fields the kernel must touch; see DESIGN.md):
+# (reads + writes of FP64 fields the kernel must touch):
 #   filter sweep (5 variables, one direction): read 5 + write 5 fields
 #   gradient sweep (u,v,w,T along one direction): read q (5) + write 4
 #   stress + flux assembly: read q 5, 12 gradients, 9 metrics + Jacobian;
@@ -40,7 +40,10 @@ STEP_BYTES_MODEL = 5048.0  # compulsory FP64 bytes per stored point per viscous 
 #   flux sweep (one direction): read the 5 fluxes of d, write its own
 #                               right-hand-side buffer (5)
 #   update: read qo 5, the three right-hand-side buffers (15), Jacobian;
-#           write qa 5
+#           write qa 5.  Viscous stages without characteristic faces run it
+#           as the zeta flux sweep (the "flux" group then has two launches):
+#           read the 5 zeta fluxes, the xi and eta buffers (10), qo 5,
+#           Jacobian; write qa 5 -- the same 208 B
 SECTION_BYTES = {"filter": 80.0, "grad": 72.0, "stress": 336.0, "flux": 80.0, "update": 208.0}
 
 

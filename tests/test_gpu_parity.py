@@ -2,7 +2,7 @@
 the same inputs.  Tolerances: single sweeps and metrics 1e-13 relative to the
 field's max (the device follows the oracle's evaluation order; only math
 library ulps differ); whole steps 1e-12 per conserved variable (max-norm
-relative), the north-star parity target."""
+relative)."""
 import numpy as np
 import pytest
 
