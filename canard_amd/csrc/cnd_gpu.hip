@@ -454,7 +454,8 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
 }
 
 // Last flux sweep of a stage fused with the RK update: sweep of the stored
-// flux component v (grid y) along d, written as the updated state qa.
+// flux component v along d, written as the updated state qa.  Grid x: five
+// workgroups (components) per tile.
 template <class G, bool MIX>
 __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_update(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
@@ -462,8 +463,12 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
     const double *__restrict__ qo, const double *__restrict__ met, double *qa, double *drva, int64_t NT, int64_t FT,
     int nk)
 {
-    SEG_PROLOGUE(ntiles)
-    const int v = blockIdx.y;
+    // the five component workgroups of a tile follow each other on one XCD,
+    // so the Jacobian they all read is fetched once into that XCD's L2
+    const int w = xcd_tile(5 * ntiles);
+    if (w >= 5 * ntiles) return;
+    extern __shared__ double lds[];
+    const int tix = w / 5, v = w % 5;
     const Tile t = tiles[tix];
     SrcField<1> src{F + v * NT, 0};
     const double dtk = inst[blks[t.b].inst].dt / double(4 - nk + 1);
@@ -1222,42 +1227,45 @@ struct Section {
 inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
 // Launch the segmented sweep K<G, MIX> (NV variables) for kind along d with
 // the shape chosen for them; the arguments after the common ones follow.
-#define SEG_CASE(i, K, MIX, NV, c, kind, d, y, ...)                                                        \
+#define SEG_CASE(i, K, MIX, NV, c, kind, d, xm, y, ...)                                                    \
     case i: {                                                                                              \
         using G = Sh##i;                                                                                   \
         const int sp_ = (c)->seg_sp[kind][d];                                                              \
         const int pi_ = p_index(G::P);                                                                     \
         const int nt_ = int((c)->hstiles[pi_][d].size());                                                  \
-        hipLaunchKernelGGL((K<G, MIX>), dim3(unsigned(xcd_width(nt_)), unsigned(y)),                       \
+        hipLaunchKernelGGL((K<G, MIX>), dim3(unsigned(xcd_width((xm) * nt_)), unsigned(y)),                \
                            dim3(unsigned((NV) * G::P * sp_)),                                              \
                            (SegShape<NV, 0, G::L, G::P, G::C, G::XI>::lds_bytes(sp_, G::SPMAX, (c)->nmax[d],     \
                                                                                 (c)->part_other[d])),              \
                            (c)->st, (c)->dblk,                                                             \
                            (c)->dtab, seg_tabs(c, G::L), (c)->dstiles[pi_][d], nt_, sp_, d, __VA_ARGS__);   \
     } break;
-#define LAUNCH_SEG(K, MIX, NV, c, kind, d, y, ...)                                                          \
+#define LAUNCH_SEG_XM(K, MIX, NV, c, kind, d, xm, y, ...)                                                       \
     do {                                                                                                   \
         switch ((c)->shape[kind][d]) {                                                                     \
-            SEG_CASE(0, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(1, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(2, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(3, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(4, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(5, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(6, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(7, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(8, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(9, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                           \
-            SEG_CASE(10, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
-            SEG_CASE(11, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
-            SEG_CASE(12, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
-            SEG_CASE(13, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
-            SEG_CASE(14, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
-            SEG_CASE(15, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
-            SEG_CASE(16, K, MIX, NV, c, kind, d, y, __VA_ARGS__)                                          \
+            SEG_CASE(0, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(1, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(2, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(3, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(4, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(5, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(6, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(7, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(8, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(9, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
+            SEG_CASE(10, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
+            SEG_CASE(11, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
+            SEG_CASE(12, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
+            SEG_CASE(13, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
+            SEG_CASE(14, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
+            SEG_CASE(15, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
+            SEG_CASE(16, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
             default: break;                                                                                \
         }                                                                                                  \
     } while (0)
+// grid x covers xm * tiles workgroups (kernels that split a tile's work
+// over xm workgroups themselves); LAUNCH_SEG: one workgroup per tile and y
+#define LAUNCH_SEG(K, MIX, NV, c, kind, d, y, ...) LAUNCH_SEG_XM(K, MIX, NV, c, kind, d, 1, y, __VA_ARGS__)
 
 inline SegTabs seg_tabs(cnd_ctx *c, int L)
 {
@@ -1382,11 +1390,11 @@ int run_flux_update(cnd_ctx *c, int nk)
     const int K = CND_KIND_FLUX, d = 2;
     const double *F = c->fl + 5 * d * c->NT;
     if (c->precision == CND_MIXED)
-        LAUNCH_SEG(k_seg_flux_update, true, 1, c, K, d, 5, c->dinst, F, c->de, c->qo, c->met, c->qa, c->drva[d],
-                   c->NT, c->FT[d], nk);
+        LAUNCH_SEG_XM(k_seg_flux_update, true, 1, c, K, d, 5, 1, c->dinst, F, c->de, c->qo, c->met, c->qa,
+                      c->drva[d], c->NT, c->FT[d], nk);
     else
-        LAUNCH_SEG(k_seg_flux_update, false, 1, c, K, d, 5, c->dinst, F, c->de, c->qo, c->met, c->qa, c->drva[d],
-                   c->NT, c->FT[d], nk);
+        LAUNCH_SEG_XM(k_seg_flux_update, false, 1, c, K, d, 5, 1, c->dinst, F, c->de, c->qo, c->met, c->qa,
+                      c->drva[d], c->NT, c->FT[d], nk);
     return check_launch(c);
 }
 
