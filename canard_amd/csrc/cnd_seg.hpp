@@ -87,8 +87,8 @@ struct SegShape {
         const int m = chunk > part ? chunk : part;
         return m > ex ? m : ex;
     }
-    // then the segment transfer values [SPM][8] and the segments' LU rows as
-    // their lanes hold them [SPM][L][5]
+    // then the segment transfer values [SPM][8] and the LU rows of the
+    // launch's SP segments as their lanes hold them [SP][L][5]
     __host__ __device__ static constexpr int ends_doubles() { return 2 * kSegEndRows * NV * P; }
     // (xi: the whole tile's sources [nmax][RS] replace the work area; once
     // the closure rows are formed, the LU rows and then the exchange buffer
@@ -97,7 +97,7 @@ struct SegShape {
     // its partner along the sweep direction.)
     __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax, bool has_part)
     {
-        const size_t tile = size_t(RS) * nmax, ex4 = size_t(4) * NV * SP * P, lu = size_t(5) * L * spm;
+        const size_t tile = size_t(RS) * nmax, ex4 = size_t(4) * NV * SP * P, lu = size_t(5) * L * SP;
         const size_t over = lu + ex4;
         if (XI)
             return sizeof(double) * ((tile > over ? tile : over) + size_t(8) * spm +
