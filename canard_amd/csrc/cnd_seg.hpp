@@ -114,6 +114,28 @@ __device__ __forceinline__ void seg_slot(int slot, int SP, int &o, int &s, int &
     else { p = slot % P; s = (slot / P) % SP; o = slot / (P * SP); }
 }
 
+// xi sweeps (XI shapes run along xi only): the P pencils of a tile are
+// consecutive rows of the block, so point e of the tile (pencil e / n, row
+// e % n) sits at g0 + e, g0 the tile's first point.  A thread walks
+// e = t, t + nthr, ... and steps (pencil p, row r) by nthr instead of
+// dividing e by n.
+struct XiWalk {
+    int p, r, dq, dr, n;
+    __device__ XiWalk(int t, int nthr, int n_) : n(n_)
+    {
+        p = t / n;
+        r = t - p * n;
+        dq = nthr / n;
+        dr = nthr - dq * n;
+    }
+    __device__ void step()
+    {
+        r += dr;
+        p += dq;
+        if (r >= n) { r -= n; ++p; }
+    }
+};
+
 // One lane's view of its segment: rows a .. a + len - 1 of the line, held
 // right-aligned in Y[kb .. L - 1] (kb = L - len); Y[0 .. kb - 1] are
 // pass-through registers.
@@ -369,11 +391,14 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     for (int m = 0; m <= 2 * H; ++m) w[m] = 0.0;
     const int nslots = C * SP * P;
     if constexpr (XI) {
-        // xi pencils are contiguous, and so are the P pencils of a tile: stage
-        // all their points at once, consecutive threads on consecutive points
+        // stage all points of the tile at once, consecutive threads on
+        // consecutive points
         constexpr int RAWX = int(sizeof(typename Src::Raw) / sizeof(double));
         constexpr int BX = (24 / RAWX) < 1 ? 1 : 24 / RAWX;
         const int tot = P * n;
+        const int64_t g0 = pt(0, 0);
+        const int xlast = npen - 1 - tile.q0;   // last pencil of the tile inside the block
+        XiWalk xw(t, nthr, n);
         __syncthreads();
         for (int e0 = t; e0 < tot; e0 += nthr * BX) {
             typename Src::Raw raw[BX];
@@ -381,12 +406,10 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
 #pragma unroll
             for (int kk = 0; kk < BX; ++kk) {
                 const int e = e0 + kk * nthr;
-                const int ec = e < tot ? e : tot - 1;
-                const int pp = ec / n, row = ec % n;
-                const bool ok = e < tot && tile.q0 + pp < npen;
-                const int ppc = tile.q0 + pp < npen ? pp : npen - 1 - tile.q0;
-                src.load(pt(ppc, row), b, raw[kk]);
-                dst[kk] = ok ? row * RS + pp : -1;
+                const bool ok = e < tot && xw.p <= xlast;
+                src.load(ok ? g0 + e : g0, b, raw[kk]);
+                dst[kk] = ok ? xw.r * RS + xw.p : -1;
+                xw.step();
             }
 #pragma unroll
             for (int kk = 0; kk < BX; ++kk) {
@@ -804,38 +827,41 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         }
         __syncthreads();
         const int tot = P * n;
+        const int64_t g0 = pt(0, 0);
+        const int xlast = npen - 1 - tile.q0;
+        XiWalk xw(t, nthr, n);
         if constexpr (Sink::kReadBack) {
             // the sink adds the results to its sources: read those back for
             // kOutBatch points at a time, all loads in flight together
             for (int e0 = t; e0 < tot; e0 += kOutBatch * nthr) {
                 double fs[kOutBatch][NV];
-                int64_t gg[kOutBatch];
-#pragma unroll
-                for (int kk = 0; kk < kOutBatch; ++kk) {
-                    const int e = e0 + kk * nthr < tot ? e0 + kk * nthr : tot - 1;
-                    const int pp = e / n, row = e % n;
-                    gg[kk] = pt(tile.q0 + pp < npen ? pp : npen - 1 - tile.q0, row);
-#pragma unroll
-                    for (int m = 0; m < NV; ++m) fs[kk][m] = sink.src(gg[kk], m);
-                }
+                int li[kOutBatch];   // LDS place of the point's results, -1: none
 #pragma unroll
                 for (int kk = 0; kk < kOutBatch; ++kk) {
                     const int e = e0 + kk * nthr;
-                    if (e >= tot || tile.q0 + e / n >= npen) continue;
-                    const int row = e % n, pp = e / n;
+                    const bool ok = e < tot && xw.p <= xlast;
+                    li[kk] = ok ? xw.r * RS + xw.p : -1;
 #pragma unroll
-                    for (int m = 0; m < NV; ++m)
-                        sink.put_raw(gg[kk], b, m, sink.fold(fs[kk][m], buf[row * RS + m * P + pp]));
+                    for (int m = 0; m < NV; ++m) fs[kk][m] = sink.src(ok ? g0 + e : g0, m);
+                    xw.step();
+                }
+#pragma unroll
+                for (int kk = 0; kk < kOutBatch; ++kk) {
+                    if (li[kk] < 0) continue;
+                    const int64_t g = g0 + (e0 + kk * nthr);
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) sink.put_raw(g, b, m, sink.fold(fs[kk][m], buf[li[kk] + m * P]));
                 }
             }
         } else {
             for (int e = t; e < tot; e += nthr) {
-                const int pp = e / n, row = e % n;
-                if (tile.q0 + pp >= npen) continue;
-                double x[NV];
+                if (xw.p <= xlast) {
+                    double x[NV];
 #pragma unroll
-                for (int m = 0; m < NV; ++m) x[m] = buf[row * RS + m * P + pp];
-                sink.store(pt(pp, row), b, x);
+                    for (int m = 0; m < NV; ++m) x[m] = buf[xw.r * RS + m * P + xw.p];
+                    sink.store(g0 + e, b, x);
+                }
+                xw.step();
             }
         }
     }
