@@ -490,22 +490,35 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     // values within about 24 doubles of registers
     constexpr int RAWD = int(sizeof(typename Src::Raw) / sizeof(double));
     constexpr int BAT = (24 / RAWD) < 1 ? 1 : ((24 / RAWD) < CPT ? 24 / RAWD : CPT);
-    // the loads of slot k of chunk c (addresses clamped into the block, so
-    // they are valid for every slot) and the LDS place of its sources
-    auto issue = [&](int c, int k, typename Src::Raw &raw, int &dst) {
+    // Slot k of a thread stages, chunk after chunk, rows r0 + c C of one
+    // segment's window [lo, hi] of pencil pb (stride st): these are the same
+    // for every chunk and computed once.  The loads of slot k of chunk c
+    // (addresses clamped into the block, so they are valid for every slot)
+    // and the LDS place of its sources (-1: nothing to stage).
+    constexpr int CPTB = (CPT + BAT - 1) / BAT * BAT;   // slots the batches below visit
+    int64_t spb[CPTB];
+    int sr0[CPTB], slo[CPTB], shi[CPTB], sdst[CPTB];
+    int64_t sst = 0;
+#pragma unroll
+    for (int k = 0; k < CPTB; ++k) {
         const int slot = k < CPT ? t + k * nthr : nslots;
         int o, ss, pp;
         seg_slot<P, C, XI>(slot < nslots ? slot : 0, SP, o, ss, pp);
         const int ssc = ss < S ? ss : S - 1;
         int sa, sl;
         seg_bounds(n, L, S, ssc, sa, sl);
-        const int row = sa - (L - sl) - H + c * C + o;
-        const bool ok = slot < nslots && ss < S && tile.q0 + pp < npen && row >= 0 && row <= N &&
-                        row >= sa - H && row < sa + sl + H;
+        const bool valid = slot < nslots && ss < S && tile.q0 + pp < npen;
+        sr0[k] = sa - (L - sl) - H + o;
+        slo[k] = sa - H > 0 ? sa - H : 0;
+        shi[k] = sa + sl + H - 1 < N ? sa + sl + H - 1 : N;
+        sdst[k] = valid ? ((o * NV) * SP + ss) * P + pp : -1;
+        pencil_of(B, d, tile.q0 + (tile.q0 + pp < npen ? pp : npen - 1 - tile.q0), spb[k], sst);
+    }
+    auto issue = [&](int c, int k, typename Src::Raw &raw, int &dst) {
+        const int row = sr0[k] + c * C;
         const int rowc = row < 0 ? 0 : (row > N ? N : row);
-        const int ppc = tile.q0 + pp < npen ? pp : npen - 1 - tile.q0;
-        src.load(pt(ppc, rowc), b, raw);
-        dst = ok ? ((o * NV) * SP + ss) * P + pp : -1;
+        src.load(spb[k] + int64_t(rowc) * sst, b, raw);
+        dst = (sdst[k] >= 0 && row >= slo[k] && row <= shi[k]) ? sdst[k] : -1;
     };
     auto deposit = [&](const typename Src::Raw &raw, int dst) {
         if (dst < 0) return;
