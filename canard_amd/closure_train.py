@@ -100,7 +100,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval-steps", type=int, default=300)
     ap.add_argument("--dt", type=float, default=0.01, help="fine time step")
-    ap.add_argument("--out", default="gpurun_out/closure")
+    ap.add_argument("--out", default="closure_out", help="folder for the weights, losses and summary")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     nf, n = a.fine, a.coarse
