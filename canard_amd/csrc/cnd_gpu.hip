@@ -1383,17 +1383,25 @@ int run_flux(cnd_ctx *c, int d, const double *cur)
     return check_launch(c);
 }
 
-// zeta flux sweep of the stored fluxes fused with the RK update into qa
-// (viscous stages without sponge, segmented zeta shape or mixed precision).
-int run_flux_update(cnd_ctx *c, int nk)
+// Viscous stages without characteristic faces or sponge fold the update
+// into the zeta flux sweep when that sweep is segmented (GCBC needs all
+// three flux derivatives before the update; the sponge stays in k_update).
+inline bool fuse_possible(const cnd_ctx *c)
+{
+    return c->fuse_update && c->any_visc && !c->any_gcbc && !c->any_sponge;
+}
+
+// zeta flux sweep of the stored fluxes fused with the RK update, written to
+// qa (the autotuner passes a scratch buffer instead).
+int run_flux_update(cnd_ctx *c, int nk, double *qa)
 {
     const int K = CND_KIND_FLUX, d = 2;
     const double *F = c->fl + 5 * d * c->NT;
     if (c->precision == CND_MIXED)
-        LAUNCH_SEG_XM(k_seg_flux_update, true, 1, c, K, d, 5, 1, c->dinst, F, c->de, c->qo, c->met, c->qa,
+        LAUNCH_SEG_XM(k_seg_flux_update, true, 1, c, K, d, 5, 1, c->dinst, F, c->de, c->qo, c->met, qa,
                       c->drva[d], c->NT, c->FT[d], nk);
     else
-        LAUNCH_SEG_XM(k_seg_flux_update, false, 1, c, K, d, 5, 1, c->dinst, F, c->de, c->qo, c->met, c->qa,
+        LAUNCH_SEG_XM(k_seg_flux_update, false, 1, c, K, d, 5, 1, c->dinst, F, c->de, c->qo, c->met, qa,
                       c->drva[d], c->NT, c->FT[d], nk);
     return check_launch(c);
 }
@@ -1440,11 +1448,7 @@ int enqueue_stage(cnd_ctx *c, int nk)
             hipLaunchKernelGGL(k_visc_flux<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt,
                                cur, c->grad, c->met, c->fl, c->NT);
     }
-    // Viscous stages without characteristic faces or sponge fold the update
-    // into the zeta flux sweep (segmented shapes only); GCBC needs all three
-    // flux derivatives before the update.
-    const bool fused = c->fuse_update && c->any_visc && !c->any_gcbc && !c->any_sponge &&
-                       (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, 2));
+    const bool fused = fuse_possible(c) && (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, 2));
     {
     Section sec(c, CND_SEC_FLUX, fused ? 2 : 3);
     for (int d = 0; d < (fused ? 2 : 3); ++d)
@@ -1463,7 +1467,7 @@ int enqueue_stage(cnd_ctx *c, int nk)
     }
     if (fused) {
         Section sec(c, CND_SEC_UPDATE, 1);
-        if ((r = run_flux_update(c, nk))) return r;
+        if ((r = run_flux_update(c, nk, c->qa))) return r;
     } else {
         Section sec(c, CND_SEC_UPDATE, 1);
         hipLaunchKernelGGL(k_update, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->qo, c->de,
@@ -1923,16 +1927,20 @@ static int tune(cnd_ctx *c)
     HIPX(c, hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(c, CND_EDEVICE, "tune: event"); }
     int r = CND_OK;
+    // the zeta flux sweep is timed as the step runs it: fused with the
+    // update (into the gradient scratch) where fusion applies
+    const bool fz = fuse_possible(c);
     auto run = [&](int kind, int d) {
         if (kind == CND_KIND_FILTER) return run_filter_field(c, d, c->qa, c->fl, 5, mix);
         if (kind == CND_KIND_GRAD) return run_grad(c, d, c->qa);
+        if (d == 2 && fz) return run_flux_update(c, 1, c->grad);
         return run_flux(c, d, c->qa);
     };
     for (int kind : {int(CND_KIND_FILTER), int(CND_KIND_GRAD), int(CND_KIND_FLUX)}) {
         if (kind == CND_KIND_GRAD && !c->any_visc) continue;
         for (int d = 0; d < 3 && !r; ++d) {
             std::vector<int> cands;
-            if (!mix) cands.push_back(kTiled);
+            if (!mix && !(kind == CND_KIND_FLUX && d == 2 && fz)) cands.push_back(kTiled);   // no fused tiled form
             for (int sh = 0; sh < kNumShapes; ++sh)
                 if (shape_sp(c, sh, d, kind_nv(c, kind)) > 0) cands.push_back(sh);
             if (cands.size() < 2) continue;
