@@ -60,6 +60,10 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same as --n)")
     ap.add_argument("--compare-mixed", type=int, default=1,
                     help="FP64 runs: also time mixed precision and report its error against FP64")
+    ap.add_argument("--compare-ensemble", type=int, default=1,
+                    help="FP64 runs: also time the 32-instance Taylor-Green ensemble (32..128^3 cells) batched "
+                         "in one context against one context per instance")
+    ap.add_argument("--ensemble-steps", type=int, default=50)
     ap.add_argument("--shapes", default="", metavar="JSON",
                     help='pin sweep shapes instead of autotuning, e.g. \'{"grad": [7, 6, 6]}\' '
                          '(shape index per direction, -1: tiled; see canard_amd.solver.shapes())')
@@ -104,6 +108,62 @@ def cpu_baseline(n, steps):
     return {"value": cells * 4 * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} time step(s) of the {n}^3 TGV (4 RK stages each) on {threads} OpenMP threads, "
                       f"{dt:.1f} s"}
+
+
+def ensemble_compare(steps, warmup):
+    """The 32-instance Taylor-Green ensemble (cases.config4: n^3 cells, n in
+    32..128, Mach and Re drawn per instance), FP64, each step replayed as a
+    hipGraph.  Batched: every instance in one context, one launch per kernel
+    over all of them.  Separate: one context (and stream) per instance, all
+    queued before any is waited for.  Both are autotuned and warmed up
+    before the timed steps."""
+    from canard_amd import cases
+    from canard_amd.solver import Case
+    blocks, grids, states, params, draws = cases.config4()
+    cells = sum(d["n"] ** 3 for d in draws)
+
+    def prepare(blks, grds, sts, prms):
+        c = Case(blks, prms)
+        c.set_grids(grds)
+        c.set_states(sts)
+        for i in range(len(prms)):
+            c.set_clock(inst=i)
+        c.use_graph(True)
+        c.step(warmup)
+        return c
+
+    def timed(ctxs):
+        for c in ctxs:
+            c.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for c in ctxs:
+                c.step(1)
+        for c in ctxs:
+            c.synchronize()
+        return time.perf_counter() - t0
+
+    print("[bench] ensemble: batched", file=sys.stderr, flush=True)
+    bat = prepare(blocks, grids, states, params)
+    dtb = timed([bat])
+    print("[bench] ensemble: one context per instance", file=sys.stderr, flush=True)
+    sep = [prepare([dict(blocks[i], inst=0, mcd=[[0, 0]] * 3)], [grids[i]], [states[i]], [params[i]])
+           for i in range(len(blocks))]
+    dts = timed(sep)
+    diff = 0.0
+    for i, c in enumerate(sep):
+        a, b = bat.state(i), c.state(0)
+        diff = max(diff, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
+        c.close()
+    bat.close()
+    return {"workload": "taylor_green_ensemble_32x(32..128)^3_periodic_viscous_rk4", "instances": len(blocks),
+            "cells": cells, "steps": steps, "graph": True,
+            "batched": {"value": cells * 4 * steps / dtb, "ms_per_step": dtb / steps * 1e3},
+            "separate": {"value": cells * 4 * steps / dts, "ms_per_step": dts / steps * 1e3},
+            "speedup_batched_vs_separate": dts / dtb,
+            "max_rel_diff_batched_vs_separate": diff,
+            "note": "value in cell-updates/s per RK stage over all instances; the two runs may pick different "
+                    "sweep shapes, so their states agree to rounding, not bitwise"}
 
 
 def main():
@@ -219,6 +279,8 @@ def main():
     }
     if mixed is not None:
         out["mixed"] = mixed
+    if prec == CND_FP64 and a.compare_ensemble:
+        out["ensemble"] = ensemble_compare(a.ensemble_steps, 3)
     print(f"[bench] GPU {dt:.2f} s; CPU baseline", file=sys.stderr, flush=True)
     if a.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n or n, a.cpu_steps)

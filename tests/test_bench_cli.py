@@ -22,5 +22,5 @@ def test_more_than_one_gpu_is_refused():
 def test_help_lists_the_contract_options():
     r = _bench("--help")
     assert r.returncode == 0
-    for opt in ("--gpus", "--steps", "--warmup", "--shapes", "--precision", "--cpu-steps"):
+    for opt in ("--gpus", "--steps", "--warmup", "--shapes", "--precision", "--cpu-steps", "--compare-ensemble"):
         assert opt in r.stdout
