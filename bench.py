@@ -61,8 +61,8 @@ def parse():
     ap.add_argument("--compare-mixed", type=int, default=1,
                     help="FP64 runs: also time mixed precision and report its error against FP64")
     ap.add_argument("--compare-ensemble", type=int, default=1,
-                    help="FP64 runs: also time the 32-instance Taylor-Green ensemble (32..128^3 cells) batched "
-                         "in one context against one context per instance")
+                    help="FP64 runs: also time Taylor-Green ensembles (32 instances of 32..128^3 cells, 64 of "
+                         "24..32^3) batched in one context against one context per instance")
     ap.add_argument("--ensemble-steps", type=int, default=50)
     ap.add_argument("--shapes", default="", metavar="JSON",
                     help='pin sweep shapes instead of autotuning, e.g. \'{"grad": [7, 6, 6]}\' '
@@ -110,16 +110,16 @@ def cpu_baseline(n, steps):
                       f"{dt:.1f} s"}
 
 
-def ensemble_compare(steps, warmup):
-    """The 32-instance Taylor-Green ensemble (cases.config4: n^3 cells, n in
-    32..128, Mach and Re drawn per instance), FP64, each step replayed as a
-    hipGraph.  Batched: every instance in one context, one launch per kernel
-    over all of them.  Separate: one context (and stream) per instance, all
-    queued before any is waited for.  Both are autotuned and warmed up
-    before the timed steps."""
+def ensemble_compare(steps, warmup, count=32, sizes=(32, 48, 64, 96, 128)):
+    """A Taylor-Green ensemble (cases.config4: count instances of n^3 cells,
+    n drawn from sizes, Mach and Re drawn per instance), FP64, each step
+    replayed as a hipGraph.  Batched: every instance in one context, one
+    launch per kernel over all of them.  Separate: one context (and stream)
+    per instance, all queued before any is waited for.  Both are autotuned
+    and warmed up before the timed steps."""
     from canard_amd import cases
     from canard_amd.solver import Case
-    blocks, grids, states, params, draws = cases.config4()
+    blocks, grids, states, params, draws = cases.config4(count=count, sizes=sizes)
     cells = sum(d["n"] ** 3 for d in draws)
 
     def prepare(blks, grds, sts, prms):
@@ -156,7 +156,8 @@ def ensemble_compare(steps, warmup):
         diff = max(diff, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
         c.close()
     bat.close()
-    return {"workload": "taylor_green_ensemble_32x(32..128)^3_periodic_viscous_rk4", "instances": len(blocks),
+    return {"workload": f"taylor_green_ensemble_{count}x({min(sizes)}..{max(sizes)})^3_periodic_viscous_rk4",
+            "instances": len(blocks),
             "cells": cells, "steps": steps, "graph": True,
             "batched": {"value": cells * 4 * steps / dtb, "ms_per_step": dtb / steps * 1e3},
             "separate": {"value": cells * 4 * steps / dts, "ms_per_step": dts / steps * 1e3},
@@ -281,6 +282,7 @@ def main():
         out["mixed"] = mixed
     if prec == CND_FP64 and a.compare_ensemble:
         out["ensemble"] = ensemble_compare(a.ensemble_steps, 3)
+        out["ensemble_small"] = ensemble_compare(a.ensemble_steps, 3, count=64, sizes=(24, 32))
     print(f"[bench] GPU {dt:.2f} s; CPU baseline", file=sys.stderr, flush=True)
     if a.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n or n, a.cpu_steps)
