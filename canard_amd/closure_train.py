@@ -15,7 +15,10 @@ Judge: from the filtered state of a held-out seed, the coarse solver runs
 eval-steps steps with and without the network (the plain compact filter is
 the fallback); both are compared with the filtered fine run over the same
 time in kinetic energy (every step) and in the kinetic-energy spectrum at
-the end.  The network is kept only if it lowers both errors.
+the end.  The network is kept only if it lowers both errors.  With several
+--alphas, the network's correction is applied scaled by the alpha whose
+coarse run stays closest to the filtered fine run in kinetic energy on a
+separate calibration seed, before the judge sees the held-out seed.
 """
 import argparse
 import json
@@ -101,6 +104,9 @@ def main(argv=None):
     ap.add_argument("--eval-steps", type=int, default=300)
     ap.add_argument("--dt", type=float, default=0.01, help="fine time step")
     ap.add_argument("--out", default="closure_out", help="folder for the weights, losses and summary")
+    ap.add_argument("--alphas", default="1",
+                    help="comma-separated scales of the network's correction tried on a calibration run (seed "
+                         "998, its own fine truth); the judge uses the one with the lowest kinetic-energy error")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     nf, n = a.fine, a.coarse
@@ -158,33 +164,64 @@ def main(argv=None):
         hist.append(dict(iters=done, train=tl, valid=vl))
         log(f"iter {done}: train {tl:.4e} valid {vl:.4e}")
 
+    def ke_err(series, truth):
+        kt = np.array(truth)
+        return float(np.sqrt(np.mean((np.array(series) - kt) ** 2)) / float(np.mean(np.abs(kt))))
+
+    def rollout(seed, alphas, tag):
+        """Coarse runs from the filtered state of `seed`: plain and with the
+        network's correction scaled by each alpha, beside the filtered fine
+        run.  Returns the kinetic-energy series and the final coarse cases."""
+        fine = tgv(nf, seed=seed)
+        plain = tgv(n, seed=0)
+        nets = [tgv(n, seed=0) for _ in alphas]
+        for c in [plain] + nets:
+            box_filter(fine, 0, c, 0)
+        ke = {"truth": [], "plain": [], "net": [[] for _ in alphas]}
+        for s in range(a.eval_steps):
+            fixed_step(plain, dtc)
+            for al, c in zip(alphas, nets):
+                fixed_step(c, dtc)
+                net.set_norm(in_mean, in_scale, al * out_scale)
+                net.apply(c, 0)
+            fixed_step(fine, dtf, r)
+            box_filter(fine, 0, target, 0)
+            ke["truth"].append(kinetic_energy(target.state(0)))
+            ke["plain"].append(kinetic_energy(plain.state(0)))
+            for i, c in enumerate(nets):
+                ke["net"][i].append(kinetic_energy(c.state(0)))
+            if s % 50 == 0:
+                log(f"{tag} step {s}: KE truth {ke['truth'][-1]:.6e} plain {ke['plain'][-1]:.6e} "
+                    f"net {' '.join(f'{v[-1]:.6e}' for v in ke['net'])}")
+        net.set_norm(in_mean, in_scale, out_scale)
+        return fine, plain, nets, ke
+
+    # the correction's scale, chosen on a calibration seed
+    alphas = [float(x) for x in a.alphas.split(",")]
+    calib = None
+    if len(alphas) > 1:
+        cf, cp, cn, cke = rollout(998, alphas, "calibration")
+        calib = {"alphas": alphas, "ke_rms_rel_err_plain": ke_err(cke["plain"], cke["truth"]),
+                 "ke_rms_rel_err_net": [ke_err(v, cke["truth"]) for v in cke["net"]]}
+        for c in [cf, cp] + cn:
+            c.close()
+        alpha = alphas[int(np.argmin(calib["ke_rms_rel_err_net"]))]
+        log(f"calibration: KE err plain {calib['ke_rms_rel_err_plain']:.3e}, per alpha "
+            f"{dict(zip(alphas, calib['ke_rms_rel_err_net']))}; alpha {alpha}")
+    else:
+        alpha = alphas[0]
+
     # judge on a held-out seed
-    fine = tgv(nf, seed=999)
-    plain, withnet = tgv(n, seed=0), tgv(n, seed=0)
-    box_filter(fine, 0, plain, 0)
-    box_filter(fine, 0, withnet, 0)
-    ke = {"truth": [], "plain": [], "net": []}
-    for s in range(a.eval_steps):
-        fixed_step(plain, dtc)
-        fixed_step(withnet, dtc)
-        net.apply(withnet, 0)
-        fixed_step(fine, dtf, r)
-        box_filter(fine, 0, target, 0)
-        ke["truth"].append(kinetic_energy(target.state(0)))
-        ke["plain"].append(kinetic_energy(plain.state(0)))
-        ke["net"].append(kinetic_energy(withnet.state(0)))
-        if s % 50 == 0:
-            log(f"judge step {s}: KE truth {ke['truth'][-1]:.6e} plain {ke['plain'][-1]:.6e} "
-                f"net {ke['net'][-1]:.6e}")
-    kt = np.array(ke["truth"])
-    err = {m: float(np.sqrt(np.mean((np.array(ke[m]) - kt) ** 2)) / float(np.mean(np.abs(kt))))
-           for m in ("plain", "net")}
+    fine, plain, (withnet,), ke = rollout(999, [alpha], "judge")
+    ke["net"] = ke["net"][0]
+    err = {m: ke_err(ke[m], ke["truth"]) for m in ("plain", "net")}
     st, sp, sn = spectrum(target.state(0)), spectrum(plain.state(0)), spectrum(withnet.state(0))
     serr = {"plain": float(np.linalg.norm(sp - st) / np.linalg.norm(st)),
             "net": float(np.linalg.norm(sn - st) / np.linalg.norm(st))}
     keep = err["net"] < err["plain"] and serr["net"] < serr["plain"]
     summary = dict(fine=nf, coarse=n, runs=a.runs, pairs=npairs, dt_fine=dtf, dt_coarse=dtc, iters=a.iters,
-                   lr=a.lr, valid_before=before, history=hist, eval_steps=a.eval_steps,
+                   lr=a.lr, valid_before=before, history=hist, eval_steps=a.eval_steps, alpha=alpha,
+                   calibration=calib,
                    t_end=float(fine.clock()["timo"]), ke_rms_rel_err=err, spectrum_rel_err=serr,
                    verdict="network kept" if keep else "fallback: plain compact filter",
                    ke_truth=ke["truth"][::10], ke_plain=ke["plain"][::10], ke_net=ke["net"][::10],
@@ -195,8 +232,8 @@ def main(argv=None):
         json.dump(summary, f)
     log(f"KE rms rel err plain {err['plain']:.3e} net {err['net']:.3e}; spectrum rel err plain "
         f"{serr['plain']:.3e} net {serr['net']:.3e}; {summary['verdict']}")
-    print(json.dumps({k: summary[k] for k in ("pairs", "iters", "ke_rms_rel_err", "spectrum_rel_err", "verdict",
-                                             "valid_before", "seconds")}))
+    print(json.dumps({k: summary[k] for k in ("pairs", "iters", "alpha", "calibration", "ke_rms_rel_err",
+                                             "spectrum_rel_err", "verdict", "valid_before", "seconds")}))
     for c in (fine, plain, withnet, coarse, target, net):
         c.close()
 
