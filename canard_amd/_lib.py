@@ -52,6 +52,9 @@ SIGNATURES = [
     ("cnd_filter", C.c_int, [C.c_void_p]),
     ("cnd_rk_stage", C.c_int, [C.c_void_p, C.c_int]),
     ("cnd_end_step", C.c_int, [C.c_void_p]),
+    ("cnd_record", C.c_int, [C.c_void_p, C.c_int]),
+    ("cnd_record_read", C.c_int, [C.c_void_p, C.c_int, _fp]),
+    ("cnd_record_reset", C.c_int, [C.c_void_p]),
     ("cnd_use_graph", C.c_int, [C.c_void_p, C.c_int]),
     ("cnd_synchronize", C.c_int, [C.c_void_p]),
     ("cnd_deriv_sweep", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp)]),

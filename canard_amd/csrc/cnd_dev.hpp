@@ -50,6 +50,7 @@ struct InstDev {
     double dt, dts, dte, timo;
     double umf[3], dudtmf[3];
     unsigned long long red[2];   // running maxima (bit patterns of >= 0 doubles)
+    double dtsum;                // recorded time since the last reset (cnd_record)
 };
 
 struct Tile { int b, q0; };

@@ -679,6 +679,57 @@ __global__ void k_end_step(InstDev *inst, int ninst)
     inst[i].timo = inst[i].timo + inst[i].dt;
 }
 
+// Recording of intermediate results (mo_canard_driver.f90:373-378): the
+// running trapezoid sum qb += dt/2 (qo + qa) over the step just taken and,
+// per instance, the recorded time dtsum += dt.
+__global__ __launch_bounds__(kPts) void k_record(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
+                                                const Tile *__restrict__ tiles, const double *__restrict__ qo,
+                                                const double *__restrict__ qa, double *__restrict__ qb, int64_t NT)
+{
+    POINT_PROLOGUE
+    const double fctr = 0.5 * inst[B.inst].dt;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) qb[m * NT + g] = qb[m * NT + g] + fctr * (qo[m * NT + g] + qa[m * NT + g]);
+}
+
+__global__ void k_record_clock(InstDev *inst, int ninst, int reset)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    inst[i].dtsum = reset ? 0.0 : inst[i].dtsum + inst[i].dt;
+}
+
+// Output variables of one block (mo_canard_driver.f90:379-400): the time
+// average qb / dtsum (the filtered state qo after the first step, n = 1)
+// as density, velocity in the fixed frame and pressure, in FP32, stored
+// variable after variable over the block's points.
+__global__ __launch_bounds__(kPts) void k_record_out(const BlkDev *__restrict__ blks,
+                                                    const InstDev *__restrict__ inst, int b,
+                                                    const double *__restrict__ qo, const double *__restrict__ qb,
+                                                    float *__restrict__ out, int64_t NT)
+{
+    const BlkDev &B = blks[b];
+    const InstDev &I = inst[B.inst];
+    const int64_t npt = int64_t(B.n[0]) * B.n[1] * B.n[2];
+    const int64_t lp = int64_t(blockIdx.x) * kPts + threadIdx.x;
+    if (lp >= npt) return;
+    const int64_t g = B.off + lp;
+    double q[5];
+    if (I.n == 1) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) q[m] = qo[m * NT + g];
+    } else {
+        const double ra0 = 1.0 / I.dtsum;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) q[m] = ra0 * qb[m * NT + g];
+    }
+    const double rr = 1.0 / q[0];
+    out[lp] = float(q[0]);
+#pragma unroll
+    for (int m = 1; m < 4; ++m) out[m * npt + lp] = float(rr * q[m] + I.umf[m - 1]);
+    out[4 * npt + lp] = float(kGm1 * (q[4] - (0.5 * rr) * ((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3])));
+}
+
 // Sponge source and RK update (mo_sponge.f90:75-88,
 // mo_canard_driver.f90:329-334): qa = qo - dt/(5-nk) * yaco * de.
 __global__ __launch_bounds__(kPts) void k_update(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
@@ -1099,6 +1150,8 @@ struct cnd_ctx {
     int seg_sp[CND_NKIND][3] = {};
     bool tuned = false, tune_on = true, tiled_only = false;
     bool fuse_update = true;   // last viscous flux sweep writes the RK update (CANARD_FUSE_UPDATE=0: separate)
+    bool recording = false;    // cnd_record: time averages accumulated in qb
+    double *qb = nullptr;
     int lds_max = 65536;   // dynamic LDS per workgroup the device allows
     double tune_ms[CND_NKIND][3] = {};
     std::vector<Tile> hpt;
@@ -1500,7 +1553,13 @@ int enqueue_stage(cnd_ctx *c, int nk)
 
 int enqueue_end(cnd_ctx *c)
 {
-    hipLaunchKernelGGL(k_end_step, dim3(unsigned((c->ninst + 63) / 64)), dim3(64), 0, c->st, c->dinst, c->ninst);
+    const unsigned ib = unsigned((c->ninst + 63) / 64);
+    hipLaunchKernelGGL(k_end_step, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst);
+    if (c->recording) {
+        hipLaunchKernelGGL(k_record, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, c->qo, c->qa,
+                           c->qb, c->NT);
+        hipLaunchKernelGGL(k_record_clock, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, 0);
+    }
     return check_launch(c);
 }
 
@@ -1781,7 +1840,8 @@ void cnd_destroy(cnd_ctx *c)
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->fl, c->met, c->ys,
                     c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->dlu32, c->dlhs, c->jc.tgt, c->jc.src,
-                    c->jc.first, c->jc.scale, c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp};
+                    c->jc.first, c->jc.scale, c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp,
+                    c->qb};
     for (void *p : ptrs) if (p) hipFree(p);
     for (int li = 0; li < kNumL; ++li) {
         if (c->dseg[li]) hipFree(c->dseg[li]);
@@ -2052,6 +2112,46 @@ int cnd_end_step(cnd_ctx *c)
     int r;
     if ((r = ready(c))) return r;
     return enqueue_end(c);
+}
+
+int cnd_record_reset(cnd_ctx *c)
+{
+    if (!c) return fail(c, CND_EINVAL, "null context");
+    if (!c->qb) return CND_OK;
+    HIPX(c, hipMemsetAsync(c->qb, 0, sizeof(double) * 5 * c->NT, c->st));
+    hipLaunchKernelGGL(k_record_clock, dim3(unsigned((c->ninst + 63) / 64)), dim3(64), 0, c->st, c->dinst, c->ninst,
+                       1);
+    return check_launch(c);
+}
+
+int cnd_record(cnd_ctx *c, int on)
+{
+    if (!c) return fail(c, CND_EINVAL, "null context");
+    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_record: a step is half done");
+    if (on && !c->qb) {
+        int r;
+        if ((r = dalloc(c, &c->qb, 5 * c->NT))) return r;
+        if ((r = cnd_record_reset(c))) return r;
+    }
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }   // the step's launches change
+    c->recording = on != 0;
+    return CND_OK;
+}
+
+int cnd_record_read(cnd_ctx *c, int b, float *out)
+{
+    if (!c || !out || b < 0 || b >= c->nblk) return fail(c, CND_EINVAL, "cnd_record_read: bad argument");
+    if (!c->qb) return fail(c, CND_ESTATE, "cnd_record_read: recording was never enabled (cnd_record)");
+    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_record_read: a step is half done");
+    const int64_t n = int64_t(c->spec[b].n[0]) * c->spec[b].n[1] * c->spec[b].n[2];
+    float *dev = reinterpret_cast<float *>(c->grad);   // gradient scratch: 12 NT doubles >= 5 n floats
+    hipLaunchKernelGGL(k_record_out, dim3(unsigned((n + kPts - 1) / kPts)), dim3(kPts), 0, c->st, c->dblk, c->dinst, b,
+                       c->qo, c->qb, dev, c->NT);
+    int r;
+    if ((r = check_launch(c))) return r;
+    HIPX(c, hipMemcpyAsync(out, dev, sizeof(float) * 5 * n, hipMemcpyDeviceToHost, c->st));
+    HIPX(c, hipStreamSynchronize(c->st));
+    return CND_OK;
 }
 
 int cnd_use_graph(cnd_ctx *c, int enable)

@@ -139,6 +139,20 @@ class Case:
     def end_step(self):
         self._check(self.L.cnd_end_step(self.h), "cnd_end_step")
 
+    def record(self, on=True):
+        """Accumulate time averages at the end of every step (cnd_record)."""
+        self._check(self.L.cnd_record(self.h, int(bool(on))), "cnd_record")
+
+    def record_read(self, b):
+        """Output variables of block b from the recorded averages: (5, points)
+        float32 -- density, three velocity components, pressure."""
+        out = np.empty(5 * self.npt[b], dtype=np.float32)
+        self._check(self.L.cnd_record_read(self.h, b, out.ctypes.data_as(C.POINTER(C.c_float))), "cnd_record_read")
+        return out.reshape(5, self.npt[b])
+
+    def record_reset(self):
+        self._check(self.L.cnd_record_reset(self.h), "cnd_record_reset")
+
     def use_graph(self, on=True):
         self._check(self.L.cnd_use_graph(self.h, int(bool(on))), "cnd_use_graph")
 

@@ -141,6 +141,22 @@ enum { CND_SEC_FILTER = 0, CND_SEC_TSTEP, CND_SEC_GRAD, CND_SEC_STRESS, CND_SEC_
 int cnd_profile(cnd_ctx *ctx, int enable);
 int cnd_profile_read(cnd_ctx *ctx, int section, double *ms, long long *launches);
 
+/* Recording of intermediate results (mo_canard_driver.f90:373-425).
+ * cnd_record(ctx, 1) starts accumulating, at the end of every step, the
+ * trapezoid sum qb += dt/2 (qo + qa) of the filtered and the new state and
+ * each instance's recorded time dtsum += dt (0 stops; the first enable
+ * allocates 5 fields and clears them).  cnd_record_read writes the output
+ * variables of block b as 5 x npts FP32 values, variable after variable in
+ * the state's point order: density, the three velocity components plus
+ * the frame velocity, and pressure, of the time average qb / dtsum -- of
+ * the filtered state qo when the instance has taken exactly one step, as
+ * the reference's first output.  cnd_record_reset clears qb and dtsum, as
+ * the reference does after writing an output.  The trigger (which steps
+ * write an output, `ndata`/`tmax`) stays with the caller.                  */
+int cnd_record(cnd_ctx *ctx, int on);
+int cnd_record_read(cnd_ctx *ctx, int block, float *out);
+int cnd_record_reset(cnd_ctx *ctx);
+
 /* Sweep-shape autotuner (no counterpart in the reference, which has no
  * device kernels).  Kinds: CND_KIND_FILTER (filter phase), CND_KIND_GRAD
  * (gradient sweeps), CND_KIND_FLUX (flux sweeps), CND_KIND_FIELD (single
