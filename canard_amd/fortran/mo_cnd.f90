@@ -87,6 +87,34 @@ module mo_cnd
          type(c_ptr), value :: ctx
          integer(c_int), value :: nsteps
       end function
+      integer(c_int) function cnd_filter(ctx) bind(C, name="cnd_filter")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function cnd_rk_stage(ctx, nk) bind(C, name="cnd_rk_stage")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nk
+      end function
+      integer(c_int) function cnd_end_step(ctx) bind(C, name="cnd_end_step")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function cnd_record(ctx, on) bind(C, name="cnd_record")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: on
+      end function
+      integer(c_int) function cnd_record_read(ctx, b, out) bind(C, name="cnd_record_read")
+         import :: c_int, c_ptr, c_float
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: b
+         real(c_float), intent(out) :: out(*)
+      end function
+      integer(c_int) function cnd_record_reset(ctx) bind(C, name="cnd_record_reset")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
       integer(c_int) function cnd_use_graph(ctx, enable) bind(C, name="cnd_use_graph")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
