@@ -1,7 +1,8 @@
 """Train and judge the learned closure on Taylor-Green runs.
 
     python -m canard_amd.closure_train [--fine 256] [--coarse 64] [--runs 4]
-        [--pairs 500] [--iters 4000] [--eval-steps 300] [--out DIR]
+        [--pairs 500] [--iters 4000] [--eval-steps 300] [--alphas 0.1,0.25,0.5,1]
+        [--out DIR]
 
 Training pairs: seeded fine-grid TGV runs (FP64, Mach 0.1, Re 1600, 1e-3
 velocity perturbation) box-filtered to the coarse grid.  For pair k the
@@ -104,7 +105,7 @@ def main(argv=None):
     ap.add_argument("--eval-steps", type=int, default=300)
     ap.add_argument("--dt", type=float, default=0.01, help="fine time step")
     ap.add_argument("--out", default="closure_out", help="folder for the weights, losses and summary")
-    ap.add_argument("--alphas", default="1",
+    ap.add_argument("--alphas", default="0.1,0.25,0.5,1",
                     help="comma-separated scales of the network's correction tried on a calibration run (seed "
                          "998, its own fine truth); the judge uses the one with the lowest kinetic-energy error")
     a = ap.parse_args(argv)
