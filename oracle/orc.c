@@ -1718,6 +1718,36 @@ void orc_penta(orc_case *C, int b, int nn, int nt, double *xl, double *xu)
     memcpy(xu, B->xu[nt][nn], sizeof(double) * 3 * n);
 }
 
+/* ---- test hooks for pinning ------------------------------------------------
+ * orc_gcbc_mats: the characteristic transforms xtq2r and xtr2q of
+ * mo_gcbc.f90:504-611 at one state q (5 conserved variables), unit normal
+ * cm and frame velocity umf; row-major 5x5 each.                          */
+void orc_gcbc_mats(const double cm[3], const double q[5], const double umf[3], double *q2r, double *r2q)
+{
+    init_constants();
+    double ri = one / q[0];
+    double p = gamm1 * (q[4] - half * ((q[1] * (q[1] * ri) + q[2] * (q[2] * ri)) + q[3] * (q[3] * ri)));
+    elem_t e;
+    eleme(&e, cm, q[0], q + 1, p, umf);
+    double xt[5][5];
+    xtq2r(&e, cm, xt);
+    memcpy(q2r, xt, sizeof xt);
+    xtr2q(&e, cm, xt);
+    memcpy(r2q, xt, sizeof xt);
+}
+
+/* orc_stage_tail: the end of an RK stage on the current state qa -- wall
+ * condition update, point and line junction averaging, surface averaging
+ * (mo_canard_driver.f90:340-355) -- with the frame velocity umf.           */
+void orc_stage_tail(orc_case *C, const double umf[3])
+{
+    for (int c = 0; c < 3; c++) C->umf[c] = umf[c];
+    wall_update(C);
+    average_junctions(C, 0);
+    average_junctions(C, 1);
+    average_surface(C);
+}
+
 void orc_destroy(orc_case *C)
 {
     if (!C) return;

@@ -78,6 +78,12 @@ void orc_penta(orc_case *c, int b, int nn, int nt, double *xl, double *xu);
 /* junction tables of one block: per corner partner count, per edge count  */
 void orc_junctions(orc_case *c, int b, int *npoint8, int *nline12);
 
+/* Test hooks: GCBC characteristic transforms at one state (row-major 5x5),
+ * and the stage tail (wall update, junction and surface averaging) applied
+ * to the current state with frame velocity umf.                            */
+void orc_gcbc_mats(const double cm[3], const double q[5], const double umf[3], double *xtq2r, double *xtr2q);
+void orc_stage_tail(orc_case *c, const double umf[3]);
+
 void orc_set_threads(int n);
 
 #ifdef __cplusplus

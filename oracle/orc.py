@@ -58,6 +58,8 @@ def lib():
         L.orc_penta.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_junctions.argtypes = [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int)]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_gcbc_mats.argtypes = [dp] * 5
+        L.orc_stage_tail.argtypes = [C.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -201,7 +203,20 @@ class Oracle:
         lib().orc_penta(self.h, b, nn, nt, _dp(xl), _dp(xu))
         return xl.reshape(n, 2), xu.reshape(n, 3)
 
+    def stage_tail(self, umf=(0.0, 0.0, 0.0)):
+        """Wall update, junction and surface averaging of the current state."""
+        u = np.ascontiguousarray(umf, dtype=np.float64)
+        lib().orc_stage_tail(self.h, _dp(u))
+
     def junctions(self, b):
         p8, l12 = (C.c_int * 8)(), (C.c_int * 12)()
         lib().orc_junctions(self.h, b, p8, l12)
         return list(p8), list(l12)
+
+
+def gcbc_mats(cm, q, umf=(0.0, 0.0, 0.0)):
+    """(xtq2r, xtr2q) 5x5 of the oracle at state q, unit normal cm."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (cm, q, umf)]
+    q2r, r2q = np.empty(25), np.empty(25)
+    lib().orc_gcbc_mats(_dp(a[0]), _dp(a[1]), _dp(a[2]), _dp(q2r), _dp(r2q))
+    return q2r.reshape(5, 5), r2q.reshape(5, 5)

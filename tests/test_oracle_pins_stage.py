@@ -1,0 +1,598 @@
+"""Pins of the oracle's stage-level functions against what the mathematics
+fixes: the characteristic transforms of the GCBC (eigenvectors of the flux
+Jacobian), the non-reflective condition on an acoustic pulse, the sponge
+(cosine ramps and the RK4 amplification of a linear relaxation), the dt
+schedule, the surface averaging and wall conditions of the stage tail, and
+the boundary filter rows (re-derived from their construction).  Each test
+says which plausible mistake in the oracle it catches."""
+import numpy as np
+import pytest
+
+from canard_amd import cases
+from helpers import oracle_for
+
+import orc
+
+GAM = 1.4
+HAMHAMM1 = 1.0 / (GAM * (GAM - 1.0))
+QINF = np.array([1.0, 0.0, 0.0, 0.0, HAMHAMM1])
+
+
+def _euler_normal_flux(q, n):
+    """Textbook Euler flux through a plane of unit normal n (lab frame)."""
+    rho, mx, my, mz, E = q
+    u = np.array([mx, my, mz]) / rho
+    p = (GAM - 1.0) * (E - 0.5 * rho * (u @ u))
+    vn = u @ n
+    return np.array([rho * vn, mx * vn + p * n[0], my * vn + p * n[1], mz * vn + p * n[2], (E + p) * vn])
+
+
+def _jacobian(q, n):
+    """dF_n/dq by complex-step differentiation (exact to rounding)."""
+    h = 1e-30
+    J = np.zeros((5, 5))
+    for j in range(5):
+        qc = q.astype(complex)
+        qc[j] += 1j * h
+        J[:, j] = np.imag(_euler_normal_flux(qc, n)) / h
+    return J
+
+
+def _random_states(k, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        rho = rng.uniform(0.7, 1.3)
+        u = rng.uniform(-0.4, 0.4, 3)
+        p = rng.uniform(0.5, 0.9)
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        q = np.array([rho, *(rho * u), p / (GAM - 1.0) + 0.5 * rho * (u @ u)])
+        out.append((q, n, u, np.sqrt(GAM * p / rho)))
+    return out
+
+
+def test_gcbc_transforms_diagonalise_the_flux_jacobian():
+    """xtq2r and xtr2q (mo_gcbc.f90:504-611) are the left and right
+    eigenvector matrices of the normal flux Jacobian: their product is the
+    identity and xtq2r A_n xtr2q = diag(vn, vn, vn, vn+c, vn-c), the order the
+    characteristic selection of gcbc_update relies on.  A wrong sign or index
+    in any of the 50 entries breaks one of the two identities."""
+    for q, n, u, c in _random_states(20, 5):
+        L, R = orc.gcbc_mats(n, q)
+        np.testing.assert_allclose(L @ R, np.eye(5), rtol=0, atol=2e-13)
+        D = L @ _jacobian(q, n) @ R
+        vn = u @ n
+        want = np.diag([vn, vn, vn, vn + c, vn - c])
+        np.testing.assert_allclose(D, want, rtol=0, atol=2e-13)
+    # the frame velocity enters the selection only, not the transforms
+    q, n, _, _ = _random_states(1, 9)[0]
+    L0, R0 = orc.gcbc_mats(n, q)
+    L1, R1 = orc.gcbc_mats(n, q, umf=(0.3, -0.2, 0.1))
+    assert np.array_equal(L0, L1) and np.array_equal(R0, R1)
+
+
+def _slab(lxi, bcx, h=1.0):
+    """lxi x 12 x 12 cells, uniform spacing h, x faces bcx, periodic y and z."""
+    blk = dict(lxi=lxi, let=12, lze=12, nbc=[bcx, [45, 45], [45, 45]], mcd=[[-1, -1], [0, 0], [0, 0]])
+    xyz = cases.gridtoy(lxi, 12, 12, doml0=0.0, doml1=lxi * h, domh=6.0 * h, span=12.0 * h)
+    return blk, xyz
+
+
+def _plane_wave_state(x, eps, centre, width, direction):
+    """Acoustic plane wave along x at rest (c = 1): p' = rho' = eps g,
+    u' = direction * eps g with g a Gaussian; direction 0 gives a standing
+    pulse that splits into two."""
+    g = eps * np.exp(-((x - centre) ** 2) / (2 * width ** 2))
+    rho = 1.0 + g
+    u = direction * g / rho
+    p = 1.0 / GAM + g
+    return np.stack([rho, rho * u, 0 * x, 0 * x, p / (GAM - 1.0) + 0.5 * rho * u * u])
+
+
+def test_nonreflective_face_zeroes_incoming_characteristics():
+    """gcbc_update on a BC 10 face (mo_gcbc.f90:261-284): after the
+    correction, the time derivative of the state at the face point, seen in
+    the characteristic variables of the face state, has no incoming part --
+    the vn+c wave at the low face and the vn-c wave at the high face -- while
+    the outgoing parts stay.  A one-dimensional smooth state (both wave
+    families present at both faces) and one first stage without the filter
+    (qo = 0, so the new state is -dt/4 yaco de) expose dq/dt directly.
+    Catches zeroing the outgoing wave, the wrong sign test or the wrong
+    face."""
+    lxi = 40
+    blk, xyz = _slab(lxi, [10, 10])
+    x = xyz[0]
+    rho = 1.0 + 0.01 * np.sin(2 * np.pi * x / lxi + 0.3)
+    u = 0.05 + 0.02 * np.cos(2 * np.pi * x / lxi + 1.1)
+    p = 1.0 / GAM + 0.008 * np.cos(4 * np.pi * x / lxi)
+    q = np.stack([rho, rho * u, 0 * x, 0 * x, p / (GAM - 1.0) + 0.5 * rho * u * u])
+    o = oracle_for([blk], [xyz], [q], dict(reoo=1000.0, viscous=0))
+    dt = 1e-3
+    o.set_clock(n=5, ndt=5, dt=dt, dts=dt, dte=dt)
+    o.stage(1)
+    rate = o.state(0) / (dt / 4)       # = dq/dt
+    npt = lxi + 1
+    for i, incoming, outgoing in ((0, 3, 4), (lxi, 4, 3)):
+        l = 6 * npt * 13 + 6 * npt + i  # a face point away from the y/z faces
+        L, _ = orc.gcbc_mats((1.0, 0.0, 0.0), q[:, l])
+        w = L @ rate[:, l]
+        scale = np.max(np.abs(w))
+        assert abs(w[incoming]) <= 1e-12 * scale, (i, w)
+        assert abs(w[outgoing]) > 1e-3 * scale, (i, w)
+    o.close()
+
+
+def test_nonreflective_faces_let_an_acoustic_pulse_out():
+    """A pressure pulse at rest splits into two plane acoustic waves that
+    leave through the BC 10 faces: less than 1% of the pulse's energy stays
+    in the box, where a reflecting condition would keep all of it."""
+    lxi = 80
+    blk, xyz = _slab(lxi, [10, 10])
+    q = _plane_wave_state(xyz[0], 1e-4, 40.0, 4.0, 0)
+    o = oracle_for([blk], [xyz], [q], dict(reoo=1000.0, viscous=0))
+
+    def energy(s):
+        return np.sum((s[4] - QINF[4]) ** 2 + (s[0] - 1.0) ** 2 + s[1] ** 2)
+
+    e0 = energy(o.state(0))
+    o.run(160)
+    assert o.clock()["timo"] > 60.0   # both halves have crossed the 40 cells to a face
+    assert energy(o.state(0)) < 1e-2 * e0
+    o.close()
+
+
+@pytest.mark.parametrize("wall,viscous", [(20, 0), (25, 1)])
+def test_wall_reflects_an_acoustic_pulse(wall, viscous):
+    """A right-running acoustic pulse meets a wall (BC 20 inviscid, BC 25
+    viscous isothermal; the GCBC wall branch mo_gcbc.f90:285-308 and
+    wall_condition_update :424-459) and comes back as a left-running pulse
+    of the same pressure sign and nearly the amplitude it keeps when it
+    travels as far in a periodic box (a rigid wall reflects u' -> -u',
+    p' -> p'; the characteristic wall condition keeps 97-98% of it).
+    Catches a wall that absorbs part of the wave (the outgoing wave dropped
+    from the reflected one keeps about 91-94%), or that reflects with the
+    wrong sign."""
+    lxi = 80
+    blk, xyz = _slab(lxi, [10, wall])
+    x = xyz[0]
+    eps = 1e-4
+    q = _plane_wave_state(x, eps, 40.0, 4.0, +1)
+    o = oracle_for([blk], [xyz], [q], dict(reoo=1e5, viscous=viscous))
+    while o.clock()["timo"] < 40.0 + 20.0:     # to the wall and 20 cells back
+        o.run(1)
+    t = o.clock()["timo"]
+    s = o.state(0).reshape(5, 13, 13, lxi + 1)[:, 6, 6, :]
+    pp = (GAM - 1.0) * (s[4] - 0.5 * s[1] ** 2 / s[0]) - 1.0 / GAM
+    uu = s[1] / s[0]
+    centre = 80.0 - (t - 40.0)
+    k = int(np.argmax(np.abs(pp)))
+    o.close()
+    # the same pulse over the same time in a periodic box
+    blk, _ = _slab(lxi, [45, 45])
+    blk["mcd"][0] = [0, 0]
+    o = oracle_for([blk], [xyz], [q], dict(reoo=1e5, viscous=viscous))
+    while o.clock()["timo"] < t:
+        o.run(1)
+    s = o.state(0).reshape(5, 13, 13, lxi + 1)[:, 6, 6, :]
+    free = np.max((GAM - 1.0) * (s[4] - 0.5 * s[1] ** 2 / s[0]) - 1.0 / GAM)
+    o.close()
+    assert abs(x[k] - centre) < 3.0, (x[k], centre)
+    assert 0.96 * free < pp[k] < 1.02 * free, pp[k] / free
+    assert -1.02 * free < uu[k] < -0.96 * free, uu[k] / free
+
+
+def _sponge_params(**kw):
+    p = dict(reoo=1000.0, viscous=1, sponge=1, szco=2.0, szth0=4.0, szth1=4.0, doml0=10.0, doml1=10.0)
+    p.update(kw)
+    return p
+
+
+def test_sponge_ramps_closed_form():
+    """spongeup / spongego (mo_sponge.f90:34-88): full strength szco on the
+    first szth0 of the domain [-doml0, doml1], a cosine ramp down to zero
+    over the next szth0, and the same at the outflow end over szth1; density
+    and energy relax with both ramps, momentum with the inflow ramp only, all
+    towards the free stream (1, 0, 0, 0, 1/(gam(gam-1))).  One first RK stage
+    on a uniform state (no flux derivative) gives q - dt/4 sigma (q - q_inf)
+    point by point.  Catches a shifted or mirrored ramp, the momentum taking
+    both ramps, or a wrong relaxation target."""
+    lxi = 40
+    blk = dict(lxi=lxi, let=12, lze=12, nbc=[[10, 10], [45, 45], [45, 45]], mcd=[[-1, -1], [0, 0], [0, 0]])
+    xyz = cases.gridtoy(lxi, 12, 12, doml0=10.0, doml1=10.0, domh=3.0, span=6.0)
+    npt = xyz.shape[1]
+    q0 = np.tile(np.array([1.02, 0.01, -0.02, 0.005, 1.80])[:, None], (1, npt))
+    prm = _sponge_params()
+    o = oracle_for([blk], [xyz], [q0], prm)
+    dt = 0.01
+    o.filter_phase()
+    o.set_clock(n=5, ndt=5, dt=dt, dts=dt, dte=dt)
+    o.stage(1)
+    q1 = o.state(0)
+    x = xyz[0]
+    lo, hi, w0, w1 = -10.0, 10.0, 4.0, 4.0
+
+    def ramp(s):   # 1 for s <= 0, cosine down to 0 at s = 1
+        return 0.5 * (1.0 + np.cos(np.pi * np.clip(s, 0.0, 1.0)))
+
+    r_in = ramp((x - (lo + w0)) / w0)
+    r_out = ramp(((hi - w1) - x) / w1)
+    sig = [2.0 * (r_in + r_out)] + [2.0 * r_in] * 3 + [2.0 * (r_in + r_out)]
+    for m in range(5):
+        want = q0[m] - (dt / 4) * sig[m] * (q0[m] - QINF[m])
+        np.testing.assert_allclose(q1[m], want, rtol=0, atol=1e-14)
+    o.close()
+
+
+def test_rk4_amplification_of_a_uniform_sponge():
+    """With the whole domain inside the full-strength inflow zone the sponge
+    is dq/dt = -szco (q - q_inf) everywhere and nothing else acts on a
+    uniform state; one time step of the four low-storage stages
+    (mo_canard_driver.f90:257-334, dt/(5-nk)) must multiply q - q_inf by the
+    RK4 polynomial 1 + z + z^2/2 + z^3/6 + z^4/24 at z = -dt szco.  Catches a
+    wrong stage coefficient, a stage reading the wrong state (qo vs qa) or a
+    sponge evaluated on the wrong state."""
+    n = 12
+    blk = dict(lxi=n, let=n, lze=n, nbc=[[45, 45]] * 3, mcd=[[0, 0]] * 3)
+    xyz = cases.gridtoy(n, n, n, doml0=0.0, doml1=6.0, domh=3.0, span=6.0)
+    npt = xyz.shape[1]
+    q0 = np.tile(np.array([1.03, 0.02, -0.01, 0.015, 1.83])[:, None], (1, npt))
+    szco, dt = 3.0, 0.1
+    # inflow zone far wider than the domain; outflow ramp starts beyond it
+    prm = _sponge_params(szco=szco, szth0=1000.0, szth1=0.5, doml0=0.0, doml1=8.0)
+    o = oracle_for([blk], [xyz], [q0], prm)
+    o.set_clock(n=5, ndt=5, dt=dt, dts=dt, dte=dt)
+    o.run(1)
+    z = -dt * szco
+    amp = 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+    q1 = o.state(0)
+    for m in range(5):
+        np.testing.assert_allclose(q1[m] - QINF[m], amp * (q0[m] - QINF[m]), rtol=1e-13, atol=1e-16)
+    o.close()
+
+
+def test_dt_schedule_sequence():
+    """dt = dts + (dte - dts) sin^2(0.05 pi (n - ndt)) with dte recomputed
+    when mod(n, 10) = 1 (mo_canard_driver.f90:204-210, 278-285): on a uniform
+    stream dt is 0 for n = 0 and 1, D sin^2(0.05 pi (n-1)) for n = 2..10 and
+    D from n = 11 on, with D the closed-form CFL limit; timo is the running
+    sum.  Catches an off-by-one in the recompute step or the ramp."""
+    n = 12
+    blk = dict(lxi=n, let=n, lze=n, nbc=[[45, 45]] * 3, mcd=[[0, 0]] * 3)
+    xyz = cases.gridtoy(n, n, n)
+    u = (0.2, -0.1, 0.05)
+    q = cases.uniform_state(xyz.shape[1], u=u)
+    o = oracle_for([blk], [xyz], [q], dict(reoo=100.0, tempoo=300.0, cfl=0.8, viscous=1))
+    h = 2 * np.pi / n
+    mu = 1.0 / 100.0   # Sutherland at T = 1
+    D0 = min(0.8 * h / (np.sqrt(3.0) + sum(abs(x) for x in u)), 0.5 * h * h / (3 * mu))
+    timo = 0.0
+    D = None
+    for step in range(25):
+        o.run(1)
+        c = o.clock()
+        if step == 1:
+            # the CFL limit itself (closed form to the metrics' 1e-13)
+            D = c["dte"]
+            assert abs(D - D0) <= 1e-12 * D0
+        if step <= 1:
+            want = 0.0
+        elif step <= 10:
+            want = D * np.sin(0.05 * np.pi * (step - 1)) ** 2
+        else:
+            want = D
+        assert c["n"] == step + 1
+        assert abs(c["dt"] - want) <= 1e-14 * (D or 1.0), (step, c["dt"], want)
+        timo += want
+        assert abs(c["timo"] - timo) <= 1e-13 * max(timo, D or 1.0)
+    o.close()
+
+
+def _two_block_x(n0=16, n1=14, let=12, lze=13, code=35):
+    b0 = dict(lxi=n0, let=let, lze=lze, nbc=[[code, code], [45, 45], [45, 45]], mcd=[[1, 1], [0, 0], [0, 0]])
+    b1 = dict(lxi=n1, let=let, lze=lze, nbc=[[code, code], [45, 45], [45, 45]], mcd=[[0, 0], [1, 1], [1, 1]])
+    g0 = cases.gridtoy(n0, let, lze, doml0=0.0, doml1=float(n0))
+    g1 = cases.gridtoy(n1, let, lze, doml0=-float(n0), doml1=float(n0 + n1))
+    return [b0, b1], [g0, g1]
+
+
+def test_surface_averaging_is_the_mean_of_both_sides():
+    """average_surface (mo_gcbc.f90:342-422) after the junction averaging:
+    a point on one interface face becomes the mean of its value and the
+    partner's pre-averaging value, points off the faces are untouched, and
+    afterwards both blocks hold the same values on their shared interface
+    planes, edges included.  (Edges where two periodic faces of one block
+    meet need not agree after one pass: the running mean over faces uses
+    snapshots taken before any face is averaged.)  Catches averaging against the wrong
+    side's snapshot, a running mean with the wrong count, or a face skipped."""
+    blocks, grids = _two_block_x()
+    rng = np.random.default_rng(4)
+    states = [cases.uniform_state(g.shape[1]) * (1 + 0.1 * rng.standard_normal((5, g.shape[1]))) for g in grids]
+    o = oracle_for(blocks, grids, states, dict(reoo=1000.0))
+    o.stage_tail()
+    after = [o.state(b) for b in range(2)]
+    shp = [(b["lze"] + 1, b["let"] + 1, b["lxi"] + 1) for b in blocks]
+    pre = [s.reshape(5, *sh) for s, sh in zip(states, shp)]
+    post = [s.reshape(5, *sh) for s, sh in zip(after, shp)]
+    # interior points unchanged
+    for a, b in zip(pre, post):
+        assert np.array_equal(a[:, 1:-1, 1:-1, 1:-1], b[:, 1:-1, 1:-1, 1:-1])
+    # face-interior points of the xi interface: block 0's last plane meets
+    # block 1's first plane and the other way round (periodic through each other)
+    for (b, ib), (p, ip) in (((0, -1), (1, 0)), ((0, 0), (1, -1))):
+        mean = 0.5 * (pre[b][:, 1:-1, 1:-1, ib] + pre[p][:, 1:-1, 1:-1, ip])
+        np.testing.assert_allclose(post[b][:, 1:-1, 1:-1, ib], mean, rtol=1e-15, atol=0)
+        np.testing.assert_allclose(post[p][:, 1:-1, 1:-1, ip], mean, rtol=1e-15, atol=0)
+    # periodic eta / zeta faces inside one block
+    for b in range(2):
+        np.testing.assert_allclose(post[b][:, 1:-1, 0, 1:-1], post[b][:, 1:-1, -1, 1:-1], rtol=1e-15)
+        mean = 0.5 * (pre[b][:, 1:-1, 0, 1:-1] + pre[b][:, 1:-1, -1, 1:-1])
+        np.testing.assert_allclose(post[b][:, 1:-1, 0, 1:-1], mean, rtol=1e-15)
+    # the two blocks' copies of the interface planes agree, edges included
+    np.testing.assert_allclose(post[0][..., -1], post[1][..., 0], rtol=1e-14)
+    np.testing.assert_allclose(post[0][..., 0], post[1][..., -1], rtol=1e-14)
+    o.close()
+
+
+def test_wall_conditions_closed_form():
+    """wall_condition_update (mo_gcbc.f90:424-459): on an inviscid wall (BC
+    20) density and momentum stay and the energy is reset so that p = rho^gam
+    / gam (isentropic from the free stream); on a viscous wall (BC 25) the
+    velocity in the lab frame vanishes (stored velocity = -umf) and T = gam
+    p / rho = 1.  Checked on the face points that no averaging touches
+    afterwards.  Catches a wrong exponent, a missing frame velocity or the
+    two walls swapped."""
+    n = 14
+    blk = dict(lxi=n, let=n, lze=n, nbc=[[45, 45], [45, 45], [20, 25]], mcd=[[0, 0], [0, 0], [-1, -1]])
+    xyz = cases.gridtoy(n, n, n)
+    rng = np.random.default_rng(8)
+    q = cases.uniform_state(xyz.shape[1], u=(0.1, 0.0, 0.0)) * (1 + 0.05 * rng.standard_normal((5, xyz.shape[1])))
+    o = oracle_for([blk], [xyz], [q], dict(reoo=1000.0))
+    umf = (0.12, -0.07, 0.03)
+    o.stage_tail(umf)
+    s = o.state(0).reshape(5, n + 1, n + 1, n + 1)
+    q0 = q.reshape(5, n + 1, n + 1, n + 1)
+    inner = (slice(1, -1), slice(1, -1))
+    # k = 0: inviscid wall
+    w = s[:, 0][(slice(None),) + inner]
+    w0 = q0[:, 0][(slice(None),) + inner]
+    np.testing.assert_array_equal(w[:4], w0[:4])
+    rho = w[0]
+    p = (GAM - 1.0) * (w[4] - 0.5 * (w[1] ** 2 + w[2] ** 2 + w[3] ** 2) / rho)
+    np.testing.assert_allclose(p, rho ** GAM / GAM, rtol=1e-14)
+    # k = n: viscous wall
+    w = s[:, -1][(slice(None),) + inner]
+    w0 = q0[:, -1][(slice(None),) + inner]
+    np.testing.assert_array_equal(w[0], w0[0])
+    for c in range(3):
+        np.testing.assert_allclose(w[1 + c] / w[0], -umf[c], rtol=1e-14)
+    p = (GAM - 1.0) * (w[4] - 0.5 * (w[1] ** 2 + w[2] ** 2 + w[3] ** 2) / w[0])
+    np.testing.assert_allclose(GAM * p / w[0], 1.0, rtol=1e-14)
+    o.close()
+
+
+def _boundary_slope_weights():
+    """Weights v_k, k = 0..5, with sum_k v_k f_k = -f'(0) (unit spacing):
+    the 10th-order central difference (weights from its moment conditions,
+    solved as a Vandermonde system) applied to the point reflection
+    f(-k) = 2 f(0) - f(k) of the data about the boundary point."""
+    k = np.arange(1, 6, dtype=np.float64)
+    M = np.array([2 * k ** (2 * m + 1) for m in range(5)])   # odd moments
+    a = np.linalg.solve(M, np.eye(5)[0])                     # f'(0) = sum a_k (f_k - f_-k)
+    v = np.zeros(6)
+    v[1:] = -2 * a
+    v[0] = 2 * a.sum()
+    return v
+
+
+def test_boundary_filter_rows_rederived():
+    """The boundary filter rows (fcbcm, mo_numerics.f90:253-302) are the
+    interior filter row with the missing outside values supplied by a ghost
+    model of exponentially decaying slope: unknowns outside decay as
+    Df(-x) = r^x Df(0), data outside continue as
+    f(-x) = f(0) - f'(0) (r^x - 1) / ln r, with f'(0) the 10th-order central
+    difference of the data reflected through the boundary point (r =
+    fltrbc).  Rebuilding the three rows from the (pinned) interior
+    coefficients that way must give the oracle's tables; a wrong rational
+    constant, sign or power of r breaks it."""
+    n = 20
+    o = orc.Oracle([dict(lxi=n, let=n, lze=n, nbc=[[10, 10]] * 3, mcd=[[-1, -1]] * 3)], fltrbc=0.9)
+    c = o.coefs()
+    r = 0.9
+    al, be, za, zb, zc = c["alphf"], c["betf"], c["fa"], c["fb"], c["fc"]
+    v = _boundary_slope_weights()
+    want_lhs = np.zeros((3, 5))
+    want_rhs = np.zeros((3, 6))   # coefficients of f_0..f_5 (rows are zero-sum)
+    for i in range(3):
+        # interior row at i over points i-3..i+3; column index = point + 3
+        row = np.zeros(10)
+        for k, z in ((1, za), (2, zb), (3, zc)):
+            row[i - k + 3] += z
+            row[i + k + 3] += z
+            row[i + 3] -= 2 * z
+        lhs = {i - 2: be, i - 1: al, i: 1.0, i + 1: al, i + 2: be}
+        rhs = np.zeros(6)
+        for pt in range(-3, 0):
+            w = row[pt + 3]
+            if w == 0:
+                continue
+            rhs[0] += w                                            # f(0)
+            rhs += w * (r ** (-pt) - 1.0) / np.log(r) * v          # - f'(0) (r^x - 1)/ln r
+        for pt in range(0, 6):
+            rhs[pt] += row[pt + 3]
+        lrow = np.zeros(5)
+        for pt, w in lhs.items():
+            if pt < 0:
+                lrow[2 - i] += w * r ** (-pt)
+            elif pt - i + 2 < 5:
+                lrow[pt - i + 2] += w
+        if i == 0:
+            rhs /= lrow[2]
+            lrow /= lrow[2]
+        want_lhs[i] = lrow
+        want_rhs[i] = rhs
+    np.testing.assert_allclose(c["albef0"], want_lhs, rtol=0, atol=1e-15)
+    offs = [[1, 2, 3, 4, 5], [0, 2, 3, 4, 5], [0, 1, 3, 4, 5]]
+    for i in range(3):
+        got = np.zeros(6)
+        for k, pt in enumerate(offs[i]):
+            got[pt] += c["fbc"][i][k]
+            got[i] -= c["fbc"][i][k]
+        np.testing.assert_allclose(got, want_rhs[i], rtol=0, atol=2e-15)
+    o.close()
+
+
+def test_one_sided_filter_equals_dense_solve():
+    """The filter sweep with one-sided boundary rows equals a dense solve of
+    the banded system built from the (pinned) coefficient tables, with the
+    far end mirrored (filte, mo_numerics.f90:930-982)."""
+    n = 20
+    o = orc.Oracle([dict(lxi=n, let=n, lze=n, nbc=[[10, 10]] * 3, mcd=[[-1, -1]] * 3)])
+    c = o.coefs()
+    N = n + 1
+    A = np.zeros((N, N))
+    B = np.zeros((N, N))
+    al, be = c["alphf"], c["betf"]
+    for i in range(3, N - 3):
+        A[i, i - 2:i + 3] = [be, al, 1, al, be]
+        for k, z in ((1, c["fa"]), (2, c["fb"]), (3, c["fc"])):
+            B[i, i - k] += z; B[i, i + k] += z; B[i, i] -= 2 * z
+    offs = [[1, 2, 3, 4, 5], [0, 2, 3, 4, 5], [0, 1, 3, 4, 5]]
+    for i in range(3):
+        for k in range(5):
+            col = i + (k - 2)
+            if 0 <= col < N:
+                A[i, col] = c["albef0"][i][k]
+                A[N - 1 - i, N - 1 - col] = c["albef0"][i][k]
+        for k, pt in enumerate(offs[i]):
+            B[i, pt] += c["fbc"][i][k]; B[i, i] -= c["fbc"][i][k]
+            B[N - 1 - i, N - 1 - pt] += c["fbc"][i][k]; B[N - 1 - i, N - 1 - i] -= c["fbc"][i][k]
+    rng = np.random.default_rng(6)
+    f = rng.standard_normal(N ** 3)
+    for d in range(3):
+        g, = o.filter(d, [f])
+        F = f.reshape((N,) * 3)
+        G = g.reshape((N,) * 3)
+        li = [F[4, 7, :], F[4, :, 7], F[:, 4, 7]][d]
+        lo = [G[4, 7, :], G[4, :, 7], G[:, 4, 7]][d]
+        np.testing.assert_allclose(lo - li, np.linalg.solve(A, B @ li), rtol=0, atol=1e-13)
+    o.close()
+
+
+
+
+@pytest.mark.parametrize("nsmf", [1, 0])
+def test_moving_frame_velocity(nsmf):
+    """movef (mo_physics.f90:144-169): with nsmf = 1 the frame moves at the
+    free-stream velocity; with nsmf = 0 its speed follows the ramp
+    R(t) = (1 - cos(pi t / timf)) / 2, advanced from the stage's start time
+    by a midpoint rule, so at the last stage's update (time timo + dt) it is
+    R(timo + dt) to within dt^3 max|R'''| / 24.  Read back through the
+    viscous wall, whose stored velocity is -umf after the update.  Catches a
+    frame velocity taken at the wrong time (off by about dt R' = 2e-2 here),
+    a first-order (Euler) advance of the ramp (off by dt^2 R'' / 2 = 1.5e-3)
+    or a wrong sign of the ramp."""
+    n = 14
+    blk = dict(lxi=n, let=n, lze=n, nbc=[[45, 45], [45, 45], [25, 25]], mcd=[[0, 0], [0, 0], [-1, -1]])
+    xyz = cases.gridtoy(n, n, n)
+    q = cases.uniform_state(xyz.shape[1])
+    timf, dt, m, t = 2.0, 0.05, 0.3, 0.4
+    o = oracle_for([blk], [xyz], [q], dict(reoo=1000.0, amach1=m, nsmf=nsmf, timf=timf))
+    o.set_clock(n=5, ndt=5, dt=dt, dts=dt, dte=dt, timo=t)
+    o.run(1)
+    s = o.state(0).reshape(5, n + 1, n + 1, n + 1)
+    frame = -s[1, 0, 5, 5] / s[0, 0, 5, 5] / m
+    if nsmf == 1:
+        assert abs(frame - 1.0) < 1e-14
+    else:
+        w = np.pi / timf
+        ramp = 0.5 * (1 - np.cos(w * (t + dt)))
+        bound = dt ** 3 * 0.5 * w ** 3 / 24
+        assert abs(frame - ramp) <= bound, (frame, ramp, bound)
+    o.close()
+
+
+def test_frame_velocity_at_the_stage_times():
+    """With nsmf = 0 a density wave at rest relative to the frame, at
+    uniform pressure, is carried by the frame velocity alone:
+    drho/dt = -umf(t) drho/dx (the fluxes use u + umf, mo_physics.f90:319-382;
+    momentum and energy fluxes stay uniform).  The four stages evaluate umf
+    at the stage start times t, t + dt/4, t + dt/3, t + dt/2
+    (mo_canard_driver.f90:261-263); integrating the same ODE with the exact
+    ramp R(t) = (1 - cos(pi t / timf)) / 2 at those times, with the oracle's
+    (pinned) filter and derivative operators, must agree to within the
+    O(dt^2) error of the scheme's one-step ramp advance per stage, i.e. to
+    dt^3 max|R''| |u_oo| max|drho/dx| / 8 on the state.  A stage evaluated at
+    a wrong time (dt/(nkrk - nk + 3) instead of dt/(nkrk - nk + 2)) is off by
+    an order of magnitude more."""
+    lxi = 32
+    blk = dict(lxi=lxi, let=12, lze=12, nbc=[[45, 45]] * 3, mcd=[[0, 0]] * 3)
+    xyz = cases.gridtoy(lxi, 12, 12, doml0=0.0, doml1=2 * np.pi, domh=3.0, span=6.0)
+    x = xyz[0]
+    rho = 1.0 + 0.01 * np.sin(x)
+    q = np.stack([rho, 0 * x, 0 * x, 0 * x, np.full_like(x, HAMHAMM1)])
+    timf, dt, m, t = 2.0, 0.05, 0.3, 0.4
+    prm = dict(reoo=1000.0, viscous=0, amach1=m, nsmf=0, timf=timf)
+    o = oracle_for([blk], [xyz], [q], prm)
+    o.set_clock(n=5, ndt=5, dt=dt, dts=dt, dte=dt, timo=t)
+    o.run(1)
+    got = o.state(0)[0]
+    # prediction with the oracle's own filter (phase order eta, zeta, xi) and
+    # xi derivative; y and z derivatives of this state vanish
+    ops = orc.Oracle([blk], **prm)
+    r0 = rho.copy()
+    for d in (1, 2, 0):
+        r0, = ops.filter(d, [r0])
+    h = 2 * np.pi / lxi
+
+    def rhs(r, tt):
+        umf = m * 0.5 * (1 - np.cos(np.pi * tt / timf))
+        return -umf * ops.deriv(0, [r])[0] / h
+
+    rk = r0.copy()
+    for k, c in enumerate((0.0, 0.25, 1.0 / 3.0, 0.5)):
+        rk = r0 + dt / (4 - k) * rhs(rk, t + c * dt)
+    ops.close()
+    err = np.max(np.abs(got - rk))
+    bound = dt ** 3 * 0.5 * (np.pi / timf) ** 2 * m * 0.01 / 8
+    assert err < bound, (err, bound)
+    o.close()
+
+
+def test_curvilinear_interface_passes_an_acoustic_pulse():
+    """Two blocks joined by a BC 30 interface (characteristic interface
+    conditions: gcbc_setup / gcbc_comm / gcbc_update, mo_gcbc.f90:133-338,
+    each wave taken from its upwind side) carry a right-running acoustic
+    pulse across the interface as one block would: after it has crossed,
+    its peak matches the single-block run to 2% and the block it left holds
+    less than 0.1% of the energy (no reflection).  Taking a wave from the
+    downwind side, or not exchanging, reflects or distorts it."""
+    n0 = 40
+    b0 = dict(lxi=n0, let=12, lze=12, nbc=[[10, 30], [45, 45], [45, 45]], mcd=[[-1, 1], [0, 0], [0, 0]])
+    b1 = dict(lxi=n0, let=12, lze=12, nbc=[[30, 10], [45, 45], [45, 45]], mcd=[[0, -1], [1, 1], [1, 1]])
+    g0 = cases.gridtoy(n0, 12, 12, doml0=0.0, doml1=float(n0), domh=6.0, span=12.0)
+    g1 = cases.gridtoy(n0, 12, 12, doml0=-float(n0), doml1=float(2 * n0), domh=6.0, span=12.0)
+    eps = 1e-4
+    qs = [_plane_wave_state(g[0], eps, 20.0, 4.0, +1) for g in (g0, g1)]
+    o = oracle_for([b0, b1], [g0, g1], qs, dict(reoo=1e5, viscous=0))
+    while o.clock()["timo"] < 30.0:
+        o.run(1)
+    t = o.clock()["timo"]
+    s0 = o.state(0).reshape(5, 13, 13, n0 + 1)[:, 6, 6, :]
+    s1 = o.state(1).reshape(5, 13, 13, n0 + 1)[:, 6, 6, :]
+    o.close()
+    blk, xyz = _slab(2 * n0, [10, 10])
+    o = oracle_for([blk], [xyz], [_plane_wave_state(xyz[0], eps, 20.0, 4.0, +1)], dict(reoo=1e5, viscous=0))
+    while o.clock()["timo"] < t:
+        o.run(1)
+    s = o.state(0).reshape(5, 13, 13, 2 * n0 + 1)[:, 6, 6, :]
+    o.close()
+
+    def pres(a):
+        return (GAM - 1.0) * (a[4] - 0.5 * a[1] ** 2 / a[0]) - 1.0 / GAM
+
+    one = np.max(pres(s))
+    assert abs(np.max(pres(s1)) - one) < 0.02 * one, (np.max(pres(s1)), one)
+    e_left = np.sum(pres(s0)[1:] ** 2)
+    e_all = np.sum(pres(s) ** 2)
+    assert e_left < 1e-3 * e_all, e_left / e_all
