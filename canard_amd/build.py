@@ -23,7 +23,8 @@ ARCH = os.environ.get("CND_ARCH", "gfx950")
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value"]
-SOURCES = ["cnd_scheme.cpp", "cnd_gpu.hip", "cnd_closure.hip"]
+SOURCES = ["cnd_scheme.cpp", "cnd_gpu.hip", "cnd_run_field.hip", "cnd_run_grad.hip", "cnd_run_flux.hip",
+           "cnd_run_update.hip", "cnd_closure.hip"]
 
 
 def _run(cmd, cwd=None):

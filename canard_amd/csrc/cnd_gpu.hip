@@ -22,460 +22,12 @@
 #include "../../include/cnd.h"
 #include "cnd_dev.hpp"
 #include "cnd_internal.hpp"
-#include "cnd_kernels.hpp"
-#include "cnd_scheme.hpp"
-#include "cnd_seg.hpp"
-#include "cnd_sweep.hpp"
+#include "cnd_ctx.hpp"
+
 
 using namespace cnd;
 
 namespace {
-
-constexpr int kLine = kTile;  // pencils per face/sweep workgroup (one wave)
-constexpr int kPts = 256;     // points per pointwise workgroup
-// Every block's fields start on a multiple of this many points (2 KiB), so
-// that workgroups reading or writing whole tiles of points touch whole
-// cache lines of every field; the padding points are never read.
-constexpr int64_t kAlignPts = 256;
-
-// ---- sources (values a sweep differentiates or filters) ---------------------
-template <int NV>
-struct SrcField {   // NV plain fields, vs apart
-    const double *a;
-    int64_t vs;
-    struct Raw { double v[NV]; };
-    static constexpr bool kDirect = true;   // variable v reads field v only
-    __device__ double field(int64_t g, int v) const { return a[v * vs + g]; }
-    __device__ void load(int64_t g, int, Raw &r) const
-    {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) r.v[k] = a[k * vs + g];
-    }
-    __device__ void finish(const Raw &r, int, double (&v)[NV]) const
-    {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] = r.v[k];
-    }
-};
-
-// u, v, w, T of a state, exactly as the reference's primitives step
-// (src/mo_canard_driver.f90:267-274).
-struct SrcPrim {
-    const double *q;
-    int64_t NT;
-    struct Raw { double q[5]; };
-    static constexpr bool kDirect = false;
-    __device__ double field(int64_t, int) const { return 0.0; }
-    __device__ void load(int64_t g, int, Raw &r) const
-    {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) r.q[m] = q[m * NT + g];
-    }
-    __device__ void finish(const Raw &r, int, double (&o)[4]) const
-    {
-        const double ri = 1.0 / r.q[0];
-        const double q2 = r.q[1], q3 = r.q[2], q4 = r.q[3];
-        const double u = q2 * ri, v = q3 * ri, w = q4 * ri;
-        const double p = kGm1 * (r.q[4] - 0.5 * ((q2 * u + q3 * v) + q4 * w));
-        o[0] = u; o[1] = v; o[2] = w;
-        o[3] = (kGam * p) * ri;
-    }
-};
-
-// The five flux components of direction d at one point
-// (src/mo_physics.f90:319-382): the inviscid part from the state q1..q5 and
-// the metric row m of d, then, with stresses t (txx, tyy, tzz, txy, tyz,
-// tzx and the three heat-flux rows), the viscous part.
-template <bool VIS>
-__device__ __forceinline__ void dir_flux(double q1, double q2, double q3, double q4, double q5, double u, double v,
-                                         double w, double p, const double (&m)[3], const double *umf,
-                                         const double *t, double (&o)[5])
-{
-    const double m0 = m[0], m1 = m[1], m2 = m[2];
-    const double s = (m0 * (u + umf[0]) + m1 * (v + umf[1])) + m2 * (w + umf[2]);
-    o[0] = q1 * s;
-    o[1] = q2 * s + m0 * p;
-    o[2] = q3 * s + m1 * p;
-    o[3] = q4 * s + m2 * p;
-    o[4] = (q5 + p) * s - p * ((umf[0] * m0 + umf[1] * m1) + umf[2] * m2);
-    if constexpr (VIS) {
-        // stress rows x (txx, txy, tzx), y (txy, tyy, tyz), z (tzx, tyz, tzz)
-        const double txx = t[0], tyy = t[1], tzz = t[2], txy = t[3], tyz = t[4], tzx = t[5];
-        o[1] = o[1] - m0 * txx; o[1] = o[1] - m1 * txy; o[1] = o[1] - m2 * tzx;
-        o[2] = o[2] - m0 * txy; o[2] = o[2] - m1 * tyy; o[2] = o[2] - m2 * tyz;
-        o[3] = o[3] - m0 * tzx; o[3] = o[3] - m1 * tyz; o[3] = o[3] - m2 * tzz;
-        o[4] = o[4] - m0 * t[6]; o[4] = o[4] - m1 * t[7]; o[4] = o[4] - m2 * t[8];
-    }
-}
-
-// Inviscid fluxes of direction d evaluated in the sweep from q and the
-// metric row of d (viscous runs differentiate the stored fluxes instead).
-struct SrcFlux {
-    const double *q;
-    const double *met;
-    const BlkDev *blks;
-    const InstDev *inst;
-    int64_t NT;
-    int d;
-    struct Raw { double q[5], m[3]; };
-    static constexpr bool kDirect = false;
-    __device__ double field(int64_t, int) const { return 0.0; }
-    __device__ void load(int64_t g, int, Raw &r) const
-    {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) r.q[k] = q[k * NT + g];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) r.m[k] = met[(3 * d + k) * NT + g];
-    }
-    __device__ void finish(const Raw &r, int b, double (&o)[5]) const
-    {
-        const double q1 = r.q[0], q2 = r.q[1], q3 = r.q[2], q4 = r.q[3], q5 = r.q[4];
-        const double ri = 1.0 / q1;
-        const double u = q2 * ri, v = q3 * ri, w = q4 * ri;
-        const double p = kGm1 * (q5 - 0.5 * ((q2 * u + q3 * v) + q4 * w));
-        dir_flux<false>(q1, q2, q3, q4, q5, u, v, w, p, r.m, inst[blks[b].inst].umf, nullptr, o);
-    }
-};
-
-// ---- sinks ----------------------------------------------------------------
-// store / put write results; fold(f, x) is the value to write for source f
-// and result x, written with put_raw by sweeps that still hold the source in
-// registers (the filter adds its correction to the source).
-template <int NV, class TO = double>
-struct SinkOut {   // out fields vs apart, stored as TO
-    TO *o;
-    int64_t vs;
-    __device__ void store(int64_t g, int, const double (&x)[NV])
-    {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) o[k * vs + g] = TO(x[k]);
-    }
-    __device__ void put(int64_t g, int, int k, double x) { o[k * vs + g] = TO(x); }
-    static constexpr bool kReadBack = false;
-    __device__ double src(int64_t, int) const { return 0.0; }
-    __device__ double fold(double, double x) const { return x; }
-    __device__ void put_raw(int64_t g, int b, int k, double x) { put(g, b, k, x); }
-    __device__ void ends(int, int, int, double) {}
-};
-
-template <int NV>
-struct SinkFilt {  // out = f + correction
-    double *o;
-    const double *f;
-    int64_t vs;
-    __device__ void store(int64_t g, int, const double (&x)[NV])
-    {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) o[k * vs + g] = f[k * vs + g] + x[k];
-    }
-    __device__ void put(int64_t g, int, int k, double x) { o[k * vs + g] = f[k * vs + g] + x; }
-    static constexpr bool kReadBack = true;   // results are added to the sources
-    __device__ double src(int64_t g, int k) const { return f[k * vs + g]; }
-    __device__ double fold(double fv, double x) const { return fv + x; }
-    __device__ void put_raw(int64_t g, int, int k, double x) { o[k * vs + g] = x; }
-    __device__ void ends(int, int, int, double) {}
-};
-
-struct SinkFlux {  // the five components of dF/dxi_d; pencil ends to drva
-    double *de, *drva;
-    int64_t NT, FT, fi0;
-    __device__ void store(int64_t g, int, const double (&x)[5])
-    {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) de[m * NT + g] = x[m];
-    }
-    __device__ void put(int64_t g, int, int m, double x) { de[m * NT + g] = x; }
-    static constexpr bool kReadBack = false;
-    __device__ double src(int64_t, int) const { return 0.0; }
-    __device__ double fold(double, double x) const { return x; }
-    __device__ void put_raw(int64_t g, int b, int m, double x) { put(g, b, m, x); }
-    __device__ void ends(int side, int lane, int m, double x) { drva[(side * 5 + m) * FT + fi0 + lane] = x; }
-};
-
-struct SinkFluxV {  // component v of dF/dxi_d; pencil ends to drva
-    double *o, *drva;
-    int v;
-    int64_t FT, fi0;
-    __device__ void store(int64_t g, int, const double (&x)[1]) { o[g] = x[0]; }
-    __device__ void put(int64_t g, int, int, double x) { o[g] = x; }
-    static constexpr bool kReadBack = false;
-    __device__ double src(int64_t, int) const { return 0.0; }
-    __device__ double fold(double, double x) const { return x; }
-    __device__ void put_raw(int64_t g, int b, int k, double x) { put(g, b, k, x); }
-    __device__ void ends(int side, int lane, int, double x) { drva[(side * 5 + v) * FT + fi0 + lane] = x; }
-};
-
-// Component v of dF/dzeta folded straight into the RK update of variable v
-// (k_update's arithmetic without the sponge, same operation order, so results
-// are bit-identical): qa = qo - dt/(5-nk) * yaco * ((de_xi + de_eta) + x).
-// Every operand is variable v at the sink's own point.  Pre holds one point's
-// operands, so sweeps can load them for several points before storing any.
-struct SinkUpdateV {
-    double *qa, *drva;
-    const double *qo, *de0, *de1, *met;
-    double dtk;   // dt / (5 - nk) of the tile's instance
-    int v;
-    int64_t NT, FT, fi0;
-    struct Pre { double s, q, r; };
-    __device__ void pre(int64_t g, Pre &o) const
-    {
-        o.s = de0[g] + de1[g];
-        o.q = qo[g];
-        o.r = dtk * met[9 * NT + g];
-    }
-    __device__ void put_pre(int64_t g, double x, const Pre &o) { qa[g] = o.q - o.r * (o.s + x); }
-    __device__ void put(int64_t g, int, int, double x)
-    {
-        Pre o;
-        pre(g, o);
-        put_pre(g, x, o);
-    }
-    __device__ void store(int64_t g, int b, const double (&x)[1]) { put(g, b, 0, x[0]); }
-    static constexpr bool kReadBack = false;
-    __device__ double src(int64_t, int) const { return 0.0; }
-    __device__ double fold(double, double x) const { return x; }
-    __device__ void put_raw(int64_t g, int b, int k, double x) { put(g, b, k, x); }
-    __device__ void ends(int side, int lane, int, double x) { drva[(side * 5 + v) * FT + fi0 + lane] = x; }
-};
-
-// ---- sweep kernels -----------------------------------------------------------
-// grid: (xcd_width(tiles of direction d), variable groups); NV waves per tile
-// of 64 pencils; C = chunk_of(d); dynamic LDS = sweep_lds_bytes<NV, C>().
-#define SWEEP_TILE_PROLOGUE(ntiles)        \
-    const int tix = xcd_tile(ntiles);      \
-    if (tix >= (ntiles)) return;           \
-    extern __shared__ double lds[];
-
-template <int C>
-__global__ __launch_bounds__(kTile) void k_fd_field(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
-                                                   const double *__restrict__ lu, const Tile *__restrict__ tiles,
-                                                   int ntiles, int d, int n45, int nmax, double *ys,
-                                                   const double *in, int64_t in_vstride, double *out,
-                                                   int64_t out_vstride)
-{
-    SWEEP_TILE_PROLOGUE(ntiles)
-    const int v = blockIdx.y;
-    SrcField<1> src{in + v * in_vstride, 0};
-    SinkOut<1> sink{out + v * out_vstride, 0};
-    sweep_tile<1, 0, C>(blks, tabs, lu, tiles[tix], int64_t(v) * ntiles + tix, d, n45, nmax, ys, src, sink, lds);
-}
-
-template <int C>
-__global__ __launch_bounds__(kTile) void k_filter_field(const BlkDev *__restrict__ blks,
-                                                       const TabDev *__restrict__ tabs,
-                                                       const double *__restrict__ lu,
-                                                       const Tile *__restrict__ tiles, int ntiles, int d, int nmax,
-                                                       double *ys, const double *in, double *out, int64_t vstride)
-{
-    SWEEP_TILE_PROLOGUE(ntiles)
-    const int v = blockIdx.y;
-    SrcField<1> src{in + v * vstride, 0};
-    SinkFilt<1> sink{out + v * vstride, in + v * vstride, 0};
-    sweep_tile<1, 1, C>(blks, tabs, lu, tiles[tix], int64_t(v) * ntiles + tix, d, 0, nmax, ys, src, sink, lds);
-}
-
-// Velocity and temperature gradients along d (the 12 derivative sweeps of
-// calc_viscous_shear_stress, src/mo_physics.f90:238-280).
-template <int C>
-__global__ __launch_bounds__(4 * kTile) void k_fd_grad(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
-                                                  const double *__restrict__ lu, const Tile *__restrict__ tiles,
-                                                  int ntiles, int d, int nmax, double *ys, const double *q,
-                                                  double *grad, int64_t NT)
-{
-    SWEEP_TILE_PROLOGUE(ntiles)
-    SrcPrim src{q, NT};
-    SinkOut<4> sink{grad + 4 * d * NT, NT};
-    sweep_tile<4, 0, C>(blks, tabs, lu, tiles[tix], tix, d, 0, nmax, ys, src, sink, lds);
-}
-
-// Flux derivatives along d (src/mo_physics.f90:385-399) into the d-th
-// right-hand-side buffer de: inviscid fluxes evaluated in the sweep ...
-template <int C>
-__global__ __launch_bounds__(5 * kTile) void k_fd_flux(const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs,
-                                                  const InstDev *__restrict__ inst, const double *__restrict__ lu,
-                                                  const Tile *__restrict__ tiles, int ntiles, int d, int nmax,
-                                                  double *ys, const double *q, const double *met, double *de,
-                                                  double *drva, int64_t NT, int64_t FT)
-{
-    SWEEP_TILE_PROLOGUE(ntiles)
-    const Tile t = tiles[tix];
-    SrcFlux src{q, met, blks, inst, NT, d};
-    SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0};
-    sweep_tile<5, 0, C>(blks, tabs, lu, t, tix, d, 0, nmax, ys, src, sink, lds);
-}
-
-// ... or stored fluxes F (component v = blockIdx.y), the viscous case.
-template <int C>
-__global__ __launch_bounds__(kTile) void k_fd_flux_field(const BlkDev *__restrict__ blks,
-                                                        const TabDev *__restrict__ tabs,
-                                                        const double *__restrict__ lu, const Tile *__restrict__ tiles,
-                                                        int ntiles, int d, int nmax, double *ys, const double *F,
-                                                        double *de, double *drva, int64_t NT, int64_t FT)
-{
-    SWEEP_TILE_PROLOGUE(ntiles)
-    const int v = blockIdx.y;
-    const Tile t = tiles[tix];
-    SrcField<1> src{F + v * NT, 0};
-    SinkFluxV sink{de + v * NT, drva, v, FT, blks[t.b].foff[d] + t.q0};
-    sweep_tile<1, 0, C>(blks, tabs, lu, t, int64_t(v) * ntiles + tix, d, 0, nmax, ys, src, sink, lds);
-}
-
-// ---- segmented sweeps (cnd_seg.hpp) ---------------------------------------
-// Compiled shapes: L points per segment (lanes per pencil = segments =
-// ceil(n / L)), P pencils per workgroup, C-point staging runs, at most SPMAX
-// segments per pencil; XI shapes stage runs along the pencil (xi sweeps),
-// the others across P neighbouring pencils (eta, zeta sweeps).  The
-// autotuner picks one per sweep kind and direction.
-template <int L_, int P_, int C_, int SPMAX_, bool XI_>
-struct SegG { static constexpr int L = L_, P = P_, C = C_, SPMAX = SPMAX_; static constexpr bool XI = XI_; };
-struct ShapeInfo { int L, P, C, SPMAX; bool XI; };
-constexpr int kNumShapes = 17;
-constexpr ShapeInfo kShapes[kNumShapes] = {
-    {17, 4, 16, 16, true},    // 0: xi default (pencils up to 272 points)
-    {33, 4, 16, 8, true},     // 1
-    {33, 16, 8, 8, false},    // 2: eta/zeta default (up to 264 points)
-    {33, 16, 4, 8, false},    // 3
-    {17, 8, 8, 16, false},    // 4
-    {33, 4, 16, 16, true},    // 5: long xi pencils (up to 528 points)
-    {33, 8, 8, 16, false},    // 6: long eta/zeta pencils (up to 528 points)
-    {17, 8, 16, 16, true},    // 7: xi, wider tiles (single-field sweeps)
-    {17, 16, 16, 16, true},   // 8
-    {17, 16, 8, 16, false},   // 9: eta/zeta, short segments, wide tiles
-    {33, 32, 8, 8, false},    // 10: eta/zeta, runs of 32 and 64 points
-    {33, 64, 8, 8, false},    // 11
-    {17, 32, 8, 16, false},   // 12
-    {17, 64, 8, 16, false},   // 13
-    {17, 32, 16, 16, true},   // 14: xi, 32-pencil tiles (single-field sweeps)
-    {17, 20, 16, 16, true},   // 15: xi, tiles sized for three workgroups per CU
-    {17, 24, 16, 16, true}};  // 16
-using Sh0 = SegG<17, 4, 16, 16, true>;
-using Sh1 = SegG<33, 4, 16, 8, true>;
-using Sh2 = SegG<33, 16, 8, 8, false>;
-using Sh3 = SegG<33, 16, 4, 8, false>;
-using Sh4 = SegG<17, 8, 8, 16, false>;
-using Sh5 = SegG<33, 4, 16, 16, true>;
-using Sh6 = SegG<33, 8, 8, 16, false>;
-using Sh7 = SegG<17, 8, 16, 16, true>;
-using Sh8 = SegG<17, 16, 16, 16, true>;
-using Sh9 = SegG<17, 16, 8, 16, false>;
-using Sh10 = SegG<33, 32, 8, 8, false>;
-using Sh11 = SegG<33, 64, 8, 8, false>;
-using Sh12 = SegG<17, 32, 8, 16, false>;
-using Sh13 = SegG<17, 64, 8, 16, false>;
-using Sh14 = SegG<17, 32, 16, 16, true>;
-using Sh15 = SegG<17, 20, 16, 16, true>;
-using Sh16 = SegG<17, 24, 16, 16, true>;
-// workgroup size bound of a kernel (shapes whose NV * P * SPMAX exceeds the
-// 1024-thread limit are rejected at launch time by shape_sp)
-__host__ __device__ constexpr int wg_bound(int x) { return x > 1024 ? 1024 : x; }
-// waves per SIMD a segmented sweep is compiled for: xi shapes of 17-point
-// segments are held to 128 VGPRs (4 waves), which their LDS also allows;
-// the others are left to the compiler
-template <class G>
-__host__ __device__ constexpr int seg_waves() { return G::XI && G::L <= 17 ? 4 : 1; }
-constexpr int kTiled = -1;   // shape index of the tiled sweep
-// pencils-per-workgroup values (tile lists) and segment lengths (tables)
-constexpr int kNumP = 7, kPs[kNumP] = {4, 8, 16, 32, 64, 20, 24};
-constexpr int kNumL = 2, kLs[kNumL] = {17, 33};
-__host__ __device__ constexpr int p_index(int P)
-{
-    return P == 4 ? 0 : (P == 8 ? 1 : (P == 16 ? 2 : (P == 32 ? 3 : (P == 64 ? 4 : (P == 20 ? 5 : 6)))));
-}
-__host__ __device__ constexpr int l_index(int L) { return L == 17 ? 0 : 1; }
-
-#define SEG_PROLOGUE(ntiles)               \
-    const int tix = xcd_tile(ntiles);      \
-    if (tix >= (ntiles)) return;           \
-    extern __shared__ double lds[];
-
-template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_fd_field(
-    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, int n45, const double *in, int64_t in_vstride, double *out, int64_t out_vstride)
-{
-    SEG_PROLOGUE(ntiles)
-    const int v = blockIdx.y;
-    SrcField<1> src{in + v * in_vstride, 0};
-    SinkOut<1> sink{out + v * out_vstride, 0};
-    seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, n45, SP, src, sink, lds);
-}
-
-template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_filter_field(
-    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, const double *in, double *out, int64_t vstride)
-{
-    SEG_PROLOGUE(ntiles)
-    const int v = blockIdx.y;
-    SrcField<1> src{in + v * vstride, 0};
-    SinkFilt<1> sink{out + v * vstride, in + v * vstride, 0};
-    seg_tile<1, 1, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
-}
-
-// Velocity and temperature gradients along d; in mixed mode they are stored
-// in FP32.
-template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX), seg_waves<G>()) void k_seg_grad(
-    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, const double *q, typename std::conditional<MIX, float, double>::type *grad,
-    int64_t NT)
-{
-    SEG_PROLOGUE(ntiles)
-    using TG = typename std::conditional<MIX, float, double>::type;
-    SrcPrim src{q, NT};
-    SinkOut<4, TG> sink{grad + 4 * d * NT, NT};
-    seg_tile<4, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
-}
-
-template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux(
-    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *q, const double *met, double *de,
-    double *drva, int64_t NT, int64_t FT)
-{
-    SEG_PROLOGUE(ntiles)
-    const Tile t = tiles[tix];
-    SrcFlux src{q, met, blks, inst, NT, d};
-    SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0};
-    seg_tile<5, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
-}
-
-template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_field(
-    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, const double *F, double *de, double *drva, int64_t NT, int64_t FT)
-{
-    SEG_PROLOGUE(ntiles)
-    const int v = blockIdx.y;
-    const Tile t = tiles[tix];
-    SrcField<1> src{F + v * NT, 0};
-    SinkFluxV sink{de + v * NT, drva, v, FT, blks[t.b].foff[d] + t.q0};
-    seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
-}
-
-// Last flux sweep of a stage fused with the RK update: sweep of the stored
-// flux component v along d, written as the updated state qa.  Grid x: five
-// workgroups (components) per tile.
-template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_update(
-    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *F, const double *de,
-    const double *__restrict__ qo, const double *__restrict__ met, double *qa, double *drva, int64_t NT, int64_t FT,
-    int nk)
-{
-    // the five component workgroups of a tile follow each other on one XCD,
-    // so the Jacobian they all read is fetched once into that XCD's L2
-    const int w = xcd_tile(5 * ntiles);
-    if (w >= 5 * ntiles) return;
-    extern __shared__ double lds[];
-    const int tix = w / 5, v = w % 5;
-    const Tile t = tiles[tix];
-    SrcField<1> src{F + v * NT, 0};
-    const double dtk = inst[blks[t.b].inst].dt / double(4 - nk + 1);
-    SinkUpdateV sink{qa + v * NT, drva, qo + v * NT, de + v * NT, de + (5 + v) * NT, met, dtk, v, NT, FT,
-                     blks[t.b].foff[d] + t.q0};
-    seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
-}
 
 // ---- pointwise kernels ------------------------------------------------------
 #define POINT_PROLOGUE                                                   \
@@ -1114,98 +666,17 @@ __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int
 
 }  // namespace
 
-// ==== context ===================================================================
-struct JuncDev {
-    int n = 0;
-    int64_t *tgt = nullptr, *src = nullptr;
-    int32_t *first = nullptr;
-    double *scale = nullptr, *tmp = nullptr;
-};
-
-struct cnd_ctx {
-    int ninst = 0, nblk = 0, precision = CND_FP64;
-    std::vector<cnd_params> prm;
-    std::vector<BlockSpec> spec;
-    std::vector<BlkDev> hblk;
-    std::vector<Tables> htab;
-    std::vector<int64_t> off;
-    int64_t NT = 0, FT[3] = {0, 0, 0};
-    long long cells = 0;
-    int64_t npts = 0;   // stored points (NT, the field stride, also counts the alignment padding)
-    hipStream_t st = nullptr;
-    BlkDev *dblk = nullptr;
-    TabDev *dtab = nullptr;
-    InstDev *dinst = nullptr;
-    double *dlu = nullptr;
-    std::vector<Tile> htiles[3];
-    Tile *dtiles[3] = {nullptr, nullptr, nullptr};
-    // segmented sweeps: tiles per direction, segments per pencil (rounded to
-    // whole waves; 0: the direction uses the tiled sweeps), homogeneous tables
-    std::vector<Tile> hstiles[kNumP][3];            // tiles of P pencils
-    Tile *dstiles[kNumP][3] = {};
-    double *dseg[kNumL] = {};                       // homogeneous solutions per L
-    // sweep shape per kind (CND_KIND_*) and direction: index into kShapes
-    // or kTiled, with the segments per pencil of the launch (seg_sp)
-    int shape[CND_NKIND][3] = {};
-    int seg_sp[CND_NKIND][3] = {};
-    bool tuned = false, tune_on = true, tiled_only = false;
-    bool fuse_update = true;   // last viscous flux sweep writes the RK update (CANARD_FUSE_UPDATE=0: separate)
-    bool recording = false;    // cnd_record: time averages accumulated in qb
-    double *qb = nullptr;
-    int lds_max = 65536;   // dynamic LDS per workgroup the device allows
-    double tune_ms[CND_NKIND][3] = {};
-    std::vector<Tile> hpt;
-    Tile *dpt = nullptr;
-    // de: right-hand side as three 5-field buffers, one per flux direction
-    // (summed in the update); fl: the 15 stored fluxes F[5 d + m] of viscous
-    // stages, scratch for metrics, single sweeps and the autotuner otherwise
-    double *qa = nullptr, *qo = nullptr, *de = nullptr, *grad = nullptr, *fl = nullptr, *met = nullptr;
-    double *ys = nullptr, *xyz = nullptr, *sponge = nullptr, *sp_par = nullptr, *tmax = nullptr;
-    int nmax[3] = {0, 0, 0};   // longest pencil per direction
-    bool part_other[3] = {};   // some block has another block as interface partner along d
-    int64_t ys_len = 0;        // doubles in the forward-value scratch
-    double *drva[3] = {}, *cmf[3] = {}, *surf[3] = {};
-    // mixed precision: FP32 gradients and stresses; FP32 LU / homogeneous
-    // tables and the FP64 banded LHS for the refinement
-    float *gradf = nullptr, *dlu32 = nullptr, *dseg32[kNumL] = {};
-    double *dlhs = nullptr;
-    JuncDev jc, je;
-    std::vector<char> grid_set;
-    bool metrics_ready = false, state_in_qo = false;
-    bool any_gcbc = false, any_wall = false, any_avg = false, any_sponge = false, any_visc = false;
-    bool use_graph = false;
-    hipGraphExec_t gexec = nullptr;
-    // section timers
-    bool prof = false, capturing = false;
-    struct Mark { int sec; hipEvent_t a, b; int launches; };
-    std::vector<Mark> marks;
-    std::vector<hipEvent_t> pool;
-    double sec_ms[CND_NSEC] = {};
-    long long sec_n[CND_NSEC] = {};
-    std::string err;
-};
-
 namespace {
 thread_local std::string g_noctx_err;
+}  // namespace
 
-int fail(cnd_ctx *c, int code, const std::string &m)
+int cnd::fail(cnd_ctx *c, int code, const std::string &m)
 {
     if (c) c->err = m; else g_noctx_err = m;
     return code;
 }
 
-#define HIPX(c, call)                                                                               \
-    do {                                                                                            \
-        hipError_t e_ = (call);                                                                     \
-        if (e_ != hipSuccess) return fail((c), CND_EDEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-template <class T>
-int dalloc(cnd_ctx *c, T **p, size_t n)
-{
-    HIPX(c, hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
-    return CND_OK;
-}
+namespace {
 
 int upload_inst(cnd_ctx *c)
 {
@@ -1227,28 +698,6 @@ int upload_inst(cnd_ctx *c)
     return CND_OK;
 }
 
-// Launch helpers ----------------------------------------------------------------
-inline dim3 tiles_grid(cnd_ctx *c, int d, int y) { return dim3(unsigned(c->htiles[d].size()), unsigned(y)); }
-inline dim3 sweep_grid(cnd_ctx *c, int d, int y) { return dim3(unsigned(xcd_width(int(c->htiles[d].size()))), unsigned(y)); }
-
-// Launch a sweep kernel template K<C> with the chunk length of direction d.
-#define LAUNCH_SWEEP(K, NV, c, d, y, block, ...)                                                           \
-    do {                                                                                                   \
-        if ((d) == 0)                                                                                      \
-            hipLaunchKernelGGL(K<kChunkXi>, sweep_grid(c, d, y), dim3(block),                              \
-                               (sweep_lds_bytes<NV, kChunkXi>()), (c)->st, __VA_ARGS__);                    \
-        else                                                                                               \
-            hipLaunchKernelGGL(K<kChunkEta>, sweep_grid(c, d, y), dim3(block),                             \
-                               (sweep_lds_bytes<NV, kChunkEta>()), (c)->st, __VA_ARGS__);                   \
-    } while (0)
-inline dim3 pts_grid(cnd_ctx *c) { return dim3(unsigned(c->hpt.size())); }
-
-int check_launch(cnd_ctx *c)
-{
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(c, CND_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
-    return CND_OK;
-}
 
 hipEvent_t pool_event(cnd_ctx *c)
 {
@@ -1277,54 +726,6 @@ struct Section {
     }
 };
 
-inline int ntiles(cnd_ctx *c, int d) { return int(c->htiles[d].size()); }
-// Launch the segmented sweep K<G, MIX> (NV variables) for kind along d with
-// the shape chosen for them; the arguments after the common ones follow.
-#define SEG_CASE(i, K, MIX, NV, c, kind, d, xm, y, ...)                                                    \
-    case i: {                                                                                              \
-        using G = Sh##i;                                                                                   \
-        const int sp_ = (c)->seg_sp[kind][d];                                                              \
-        const int pi_ = p_index(G::P);                                                                     \
-        const int nt_ = int((c)->hstiles[pi_][d].size());                                                  \
-        hipLaunchKernelGGL((K<G, MIX>), dim3(unsigned(xcd_width((xm) * nt_)), unsigned(y)),                \
-                           dim3(unsigned((NV) * G::P * sp_)),                                              \
-                           (SegShape<NV, 0, G::L, G::P, G::C, G::XI>::lds_bytes(sp_, G::SPMAX, (c)->nmax[d],     \
-                                                                                (c)->part_other[d])),              \
-                           (c)->st, (c)->dblk,                                                             \
-                           (c)->dtab, seg_tabs(c, G::L), (c)->dstiles[pi_][d], nt_, sp_, d, __VA_ARGS__);   \
-    } break;
-#define LAUNCH_SEG_XM(K, MIX, NV, c, kind, d, xm, y, ...)                                                       \
-    do {                                                                                                   \
-        switch ((c)->shape[kind][d]) {                                                                     \
-            SEG_CASE(0, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(1, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(2, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(3, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(4, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(5, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(6, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(7, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(8, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(9, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                       \
-            SEG_CASE(10, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
-            SEG_CASE(11, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
-            SEG_CASE(12, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
-            SEG_CASE(13, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
-            SEG_CASE(14, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
-            SEG_CASE(15, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
-            SEG_CASE(16, K, MIX, NV, c, kind, d, xm, y, __VA_ARGS__)                                      \
-            default: break;                                                                                \
-        }                                                                                                  \
-    } while (0)
-// grid x covers xm * tiles workgroups (kernels that split a tile's work
-// over xm workgroups themselves); LAUNCH_SEG: one workgroup per tile and y
-#define LAUNCH_SEG(K, MIX, NV, c, kind, d, y, ...) LAUNCH_SEG_XM(K, MIX, NV, c, kind, d, 1, y, __VA_ARGS__)
-
-inline SegTabs seg_tabs(cnd_ctx *c, int L)
-{
-    return SegTabs{c->dlu, c->dseg[l_index(L)], c->dlhs, c->dlu32, c->dseg32[l_index(L)]};
-}
-inline bool seg_on(cnd_ctx *c, int kind, int d) { return c->shape[kind][d] != kTiled; }
 
 // Segments per pencil a launch of shape sh along d needs (the largest over
 // blocks, rounded up to whole waves per variable where that fits), or 0 when
@@ -1356,107 +757,6 @@ int shape_sp(cnd_ctx *c, int sh, int d, int nv)
     if (nv * G.P * sp > 1024) return 0;
     if (seg_lds_bytes(G, nv, sp, c->nmax[d], c->part_other[d]) > size_t(c->lds_max)) return 0;
     return sp;
-}
-// Variables per workgroup of a sweep kind (viscous flux sweeps run one
-// workgroup column per component of the stored fluxes).
-inline int kind_nv(const cnd_ctx *c, int kind)
-{
-    return kind == CND_KIND_GRAD ? 4 : (kind == CND_KIND_FLUX && !c->any_visc ? 5 : 1);
-}
-
-// Filter (kind CND_KIND_FILTER) and plain derivative (CND_KIND_FIELD) sweeps
-// of nvar fields; mix: FP32 solves with FP64 refinement (segmented only).
-int run_filter_field(cnd_ctx *c, int d, const double *in, double *out, int nvar, bool mix = false)
-{
-    const int K = CND_KIND_FILTER;
-    if (mix)
-        LAUNCH_SEG(k_seg_filter_field, true, 1, c, K, d, nvar, in, out, c->NT);
-    else if (seg_on(c, K, d))
-        LAUNCH_SEG(k_seg_filter_field, false, 1, c, K, d, nvar, in, out, c->NT);
-    else
-        LAUNCH_SWEEP(k_filter_field, 1, c, d, nvar, kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d,
-                     c->nmax[d], c->ys, in, out, c->NT);
-    return check_launch(c);
-}
-
-int run_fd_field(cnd_ctx *c, int d, int n45, const double *in, int64_t ivs, double *out, int64_t ovs, int nvar,
-                 bool mix = false)
-{
-    const int K = CND_KIND_FIELD;
-    if (mix)
-        LAUNCH_SEG(k_seg_fd_field, true, 1, c, K, d, nvar, n45, in, ivs, out, ovs);
-    else if (seg_on(c, K, d))
-        LAUNCH_SEG(k_seg_fd_field, false, 1, c, K, d, nvar, n45, in, ivs, out, ovs);
-    else
-        LAUNCH_SWEEP(k_fd_field, 1, c, d, nvar, kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d, n45,
-                     c->nmax[d], c->ys, in, ivs, out, ovs);
-    return check_launch(c);
-}
-
-// Gradient sweep along d of the state cur (kind CND_KIND_GRAD).
-int run_grad(cnd_ctx *c, int d, const double *cur)
-{
-    const int K = CND_KIND_GRAD;
-    if (c->precision == CND_MIXED)
-        LAUNCH_SEG(k_seg_grad, true, 4, c, K, d, 1, cur, c->gradf, c->NT);
-    else if (seg_on(c, K, d))
-        LAUNCH_SEG(k_seg_grad, false, 4, c, K, d, 1, cur, c->grad, c->NT);
-    else
-        LAUNCH_SWEEP(k_fd_grad, 4, c, d, 1, 4 * kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d,
-                     c->nmax[d], c->ys, cur, c->grad, c->NT);
-    return check_launch(c);
-}
-
-// Flux sweep along d into the d-th right-hand-side buffer (kind
-// CND_KIND_FLUX): of the stored fluxes (viscous runs, one workgroup column
-// per component) or of the inviscid fluxes of the state cur.
-int run_flux(cnd_ctx *c, int d, const double *cur)
-{
-    const int K = CND_KIND_FLUX;
-    const bool mix = c->precision == CND_MIXED;
-    double *de = c->de + 5 * d * c->NT;
-    if (c->any_visc) {
-        const double *F = c->fl + 5 * d * c->NT;
-        if (mix)
-            LAUNCH_SEG(k_seg_flux_field, true, 1, c, K, d, 5, F, de, c->drva[d], c->NT, c->FT[d]);
-        else if (seg_on(c, K, d))
-            LAUNCH_SEG(k_seg_flux_field, false, 1, c, K, d, 5, F, de, c->drva[d], c->NT, c->FT[d]);
-        else
-            LAUNCH_SWEEP(k_fd_flux_field, 1, c, d, 5, kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d,
-                         c->nmax[d], c->ys, F, de, c->drva[d], c->NT, c->FT[d]);
-    } else {
-        if (mix)
-            LAUNCH_SEG(k_seg_flux, true, 5, c, K, d, 1, c->dinst, cur, c->met, de, c->drva[d], c->NT, c->FT[d]);
-        else if (seg_on(c, K, d))
-            LAUNCH_SEG(k_seg_flux, false, 5, c, K, d, 1, c->dinst, cur, c->met, de, c->drva[d], c->NT, c->FT[d]);
-        else
-            LAUNCH_SWEEP(k_fd_flux, 5, c, d, 1, 5 * kTile, c->dblk, c->dtab, c->dinst, c->dlu, c->dtiles[d],
-                         ntiles(c, d), d, c->nmax[d], c->ys, cur, c->met, de, c->drva[d], c->NT, c->FT[d]);
-    }
-    return check_launch(c);
-}
-
-// Viscous stages without characteristic faces or sponge fold the update
-// into the zeta flux sweep when that sweep is segmented (GCBC needs all
-// three flux derivatives before the update; the sponge stays in k_update).
-inline bool fuse_possible(const cnd_ctx *c)
-{
-    return c->fuse_update && c->any_visc && !c->any_gcbc && !c->any_sponge;
-}
-
-// zeta flux sweep of the stored fluxes fused with the RK update, written to
-// qa (the autotuner passes a scratch buffer instead).
-int run_flux_update(cnd_ctx *c, int nk, double *qa)
-{
-    const int K = CND_KIND_FLUX, d = 2;
-    const double *F = c->fl + 5 * d * c->NT;
-    if (c->precision == CND_MIXED)
-        LAUNCH_SEG_XM(k_seg_flux_update, true, 1, c, K, d, 5, 1, c->dinst, F, c->de, c->qo, c->met, qa,
-                      c->drva[d], c->NT, c->FT[d], nk);
-    else
-        LAUNCH_SEG_XM(k_seg_flux_update, false, 1, c, K, d, 5, 1, c->dinst, F, c->de, c->qo, c->met, qa,
-                      c->drva[d], c->NT, c->FT[d], nk);
-    return check_launch(c);
 }
 
 // Filter phase (mo_canard_driver.f90:233-252): eta, zeta, xi sweeps of all
