@@ -12,7 +12,7 @@ state resident in HBM.
 Prints one JSON line.  value = nominal cells x 4 RK stages x K / seconds.
 With --compare-mixed 1 (the default for an FP64 run) the same workload is
 then run in mixed precision (FP32 line solves with FP64 refinement, FP32
-gradients and stresses) for the same W + K steps; the line's "mixed" object
+gradients, fluxes and right-hand sides) for the same W + K steps; the line's "mixed" object
 gives its rate, its speed-up over FP64 and the max-norm relative error of
 each conserved variable against the FP64 state after W + K steps.
 """
@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--graph", type=int, default=0,
                     help="replay whole steps as a hipGraph (section timers then stay empty)")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same as --n)")
     ap.add_argument("--compare-mixed", type=int, default=1,
                     help="FP64 runs: also time mixed precision and report its error against FP64")
@@ -89,25 +89,55 @@ def traffic_from_profiles(section, n):
     return None
 
 
+def host_description():
+    """The host the CPU baseline runs on: logical CPUs of the machine and of
+    this process's affinity mask, the CPU model (lscpu), and the OpenMP
+    thread settings of the environment."""
+    import platform
+    import subprocess
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"nproc": os.cpu_count(), "affinity_cpus": affinity, "model": model or platform.processor(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES")}
+
+
 def cpu_baseline(n, steps):
     """Times the CPU oracle (C restatement of the reference, OpenMP over
-    pencils) on the same TGV workload, on this host's cores."""
+    pencils) on the same TGV workload, on this host's cores: as many OpenMP
+    threads as OMP_NUM_THREADS names (the CPU share a job is given), else one
+    per CPU of the process's affinity mask.  Steps 0 and 1 of a run have
+    dt = 0 but do the full work, so every timed step costs the same."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import subprocess
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=False)
     from helpers import oracle_for, tgv_case
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    host = host_description()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or host["affinity_cpus"] or os.cpu_count() or 1
     blocks, grids, states, params = tgv_case(n, pert=1e-3)
     o = oracle_for(blocks, grids, states, params, threads=threads)
+    o.run(1)   # first touch of the oracle's arrays
     t0 = time.perf_counter()
     o.run(steps)
     dt = time.perf_counter() - t0
     o.close()
     cells = n ** 3
     return {"value": cells * 4 * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} time step(s) of the {n}^3 TGV (4 RK stages each) on {threads} OpenMP threads, "
-                      f"{dt:.1f} s"}
+            "sample": f"{steps} time steps of the {n}^3 TGV (4 RK stages each) after one untimed step, on "
+                      f"{threads} OpenMP threads, {dt:.1f} s",
+            "host": host}
 
 
 def ensemble_compare(steps, warmup, count=32, sizes=(32, 48, 64, 96, 128)):
@@ -238,9 +268,10 @@ def main():
         mixed = {"value": cells * 4 * a.steps / dtm, "ms_per_step": dtm / a.steps * 1e3,
                  "speedup_vs_fp64": dt / dtm, "steps_compared": a.warmup + a.steps,
                  "max_rel_err_per_variable": err, "error_target": 1e-6,
-                 "note": "FP32 line solves with one FP64 residual refinement per sweep; gradients and stresses "
-                         "stored in FP32; state, metrics, RK update and GCBC in FP64. Error: max-norm over the "
-                         "field, momentum components against the largest momentum component."}
+                 "note": "FP32 line solves with one FP64 residual refinement per sweep; gradients, stored fluxes "
+                         "and right-hand-side buffers in FP32; state, metrics, Jacobian, RK update and GCBC in "
+                         "FP64. Error: max-norm over the field, momentum components against the largest momentum "
+                         "component."}
         del qm
     value = cells * 4 * a.steps / dt
     # dominant kernel group by measured time

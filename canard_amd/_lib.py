@@ -12,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.environ.get("CANARD_LIB") or os.path.join(HERE, "libcnd.so")
 
 CND_FP64, CND_MIXED = 0, 1
+# grid generators (include/cnd.h CND_GRID_*)
+GRIDS = {"toy": 0, "toy_multi": 1, "sheared": 2}
 # section timers (include/cnd.h CND_SEC_*)
 SECTIONS = ["filter", "tstep", "grad", "stress", "flux", "gcbc", "update", "avg"]
 # sweep kinds of the autotuner (include/cnd.h CND_KIND_*)
@@ -43,6 +45,8 @@ SIGNATURES = [
     ("cnd_last_error", C.c_char_p, [C.c_void_p]),
     ("cnd_set_grid", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("cnd_init_metrics", C.c_int, [C.c_void_p]),
+    ("cnd_make_grid", C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int]),
+    ("cnd_get_grid", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("cnd_get_metrics", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]),
     ("cnd_set_state", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("cnd_get_state", C.c_int, [C.c_void_p, C.c_int, _dp]),
@@ -55,6 +59,13 @@ SIGNATURES = [
     ("cnd_record", C.c_int, [C.c_void_p, C.c_int]),
     ("cnd_record_read", C.c_int, [C.c_void_p, C.c_int, _fp]),
     ("cnd_record_reset", C.c_int, [C.c_void_p]),
+    ("cnd_output_schedule", C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int]),
+    ("cnd_output_due", C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp]),
+    ("cnd_output_write", C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int]),
+    ("cnd_output_grid", C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
+    ("cnd_output_wait", C.c_int, [C.c_void_p]),
+    ("cnd_write_plt", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_int, _ip,
+                                C.c_double, C.POINTER(_fp)]),
     ("cnd_use_graph", C.c_int, [C.c_void_p, C.c_int]),
     ("cnd_synchronize", C.c_int, [C.c_void_p]),
     ("cnd_deriv_sweep", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp)]),

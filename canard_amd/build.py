@@ -24,7 +24,8 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch=
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value"]
 SOURCES = ["cnd_scheme.cpp", "cnd_gpu.hip", "cnd_run_field.hip", "cnd_run_grad.hip", "cnd_run_flux.hip",
-           "cnd_run_update.hip", "cnd_closure.hip"]
+           "cnd_run_update.hip", "cnd_output.hip", "cnd_grid.hip",
+           "cnd_closure.hip"]
 
 
 def _run(cmd, cwd=None):

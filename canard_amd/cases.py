@@ -36,6 +36,27 @@ def gridtoy(lxi, let, lze, doml0=0.0, doml1=2 * np.pi, domh=np.pi, span=2 * np.p
     return np.stack([X.ravel(), Y.ravel(), Z.ravel()])
 
 
+def gridtoy_multi(mb, lxi0, lxi1, lxi2, let0, lze0, doml0=0.0, doml1=2 * np.pi, domh=np.pi, span=2 * np.pi):
+    """Block mb (0..5) of gridtoy_multi (src/mo_gridgen.f90:569-654): a 3 x 2
+    arrangement of blocks with lxi0 | lxi1 | lxi2 cells along x and let0
+    cells along y each, points spaced (doml1 + doml0)/(lxi0 + lxi1 + lxi2)
+    in x and domh/let0 in y; z as gridtoy."""
+    lxi = (lxi0, lxi1, lxi2)[mb % 3]
+    col = mb % 3
+    i0 = (0, lxi0 + 1, lxi0 + lxi1 + 2)[col]
+    j0 = 0 if mb < 3 else let0 + 1
+    i = np.arange(lxi + 1, dtype=np.float64) + i0
+    j = np.arange(let0 + 1, dtype=np.float64) + j0
+    k = np.arange(lze0 + 1, dtype=np.float64)
+    ra0 = (doml1 + doml0) / (lxi0 + lxi1 + lxi2)
+    ra1 = domh / (2.0 * let0) * 2.0
+    x = -doml0 + (i - col) * ra0
+    y = -domh + (j - (0 if mb < 3 else 1)) * ra1
+    z = span * ((lze0 - k) / lze0 - 0.5)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()])
+
+
 def splitmix64(x):
     """SplitMix64 finaliser on uint64 numpy arrays (counter-based RNG)."""
     x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -53,12 +74,14 @@ def uniform01(seed, idx):
     return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
-def tgv_state(xyz, shape, mach=0.1, pert=0.0, seed=20261202):
+def tgv_state(xyz, shape, mach=0.1, pert=0.0, seed=20261202, modes=0.0, mode_seed=0):
     """Taylor-Green vortex: u = M sin x cos y cos z, v = -M cos x sin y cos z,
     w = 0, p = 1/gam + M^2/16 (cos 2x + cos 2y)(cos 2z + 2), rho = gam p
     (T = gam p / rho = 1).  pert > 0 adds a seeded uniform velocity
     perturbation of amplitude pert*M, keyed by the periodic point index
-    (i mod n, j mod n, k mod n) so duplicated periodic points agree."""
+    (i mod n, j mod n, k mod n) so duplicated periodic points agree; modes >
+    0 adds resolved-scale Fourier modes (resolved_modes) of relative
+    amplitude modes drawn from mode_seed."""
     x, y, z = xyz
     u = mach * np.sin(x) * np.cos(y) * np.cos(z)
     v = -mach * np.cos(x) * np.sin(y) * np.cos(z)
@@ -70,10 +93,32 @@ def tgv_state(xyz, shape, mach=0.1, pert=0.0, seed=20261202):
         base = ((kk.astype(np.int64) * (ny - 1) + jj) * (nx - 1) + ii) * 3
         for c, arr in enumerate((u, v, w)):
             arr += pert * mach * (2.0 * uniform01(seed, base + c) - 1.0)
+    if modes > 0.0:
+        du = resolved_modes(xyz, mach, modes, mode_seed)
+        u += du[0]
+        v += du[1]
+        w += du[2]
     p = 1.0 / GAM + (mach * mach / 16.0) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
     rho = GAM * p
     e = p / (GAM - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
     return np.stack([rho, rho * u, rho * v, rho * w, e])
+
+
+def resolved_modes(xyz, mach, amp, seed, nmodes=8, kmax=3):
+    """Velocity perturbation at resolved scales on the [0, 2 pi]-periodic
+    box: per component, nmodes Fourier modes sin(k . x + phi) with integer
+    wavevectors |k_i| <= kmax and random phases, each of amplitude
+    amp * mach / sqrt(nmodes), all drawn from seed.  Returns (3, points)."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((3, xyz.shape[1]))
+    for c in range(3):
+        for _ in range(nmodes):
+            k = rng.integers(-kmax, kmax + 1, size=3)
+            if not k.any():
+                k[c] = 1
+            phi = rng.uniform(0.0, 2 * np.pi)
+            out[c] += amp * mach / np.sqrt(nmodes) * np.sin(k[0] * xyz[0] + k[1] * xyz[1] + k[2] * xyz[2] + phi)
+    return out
 
 
 def uniform_state(npt, u=(0.0, 0.0, 0.0), rho=1.0, p=1.0 / GAM):

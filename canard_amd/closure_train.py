@@ -5,14 +5,18 @@
         [--out DIR]
 
 Training pairs: seeded fine-grid TGV runs (FP64, Mach 0.1, Re 1600, 1e-3
-velocity perturbation) box-filtered to the coarse grid.  For pair k the
+grid-scale velocity perturbation plus resolved-scale Fourier modes of 5% of
+the Mach number drawn from the seed, so every seed is a different flow)
+box-filtered to the coarse grid.  For pair k the
 coarse solver takes one step of dt_c = r dt_f from the filtered state F_k;
 the input is that coarse state C_k and the target F_{k+1} - C_k, where
 F_{k+1} is the fine solution after r fine steps, filtered.  The time step
 is held fixed (dt_f, dt_c) through the clock so both grids cover the same
 interval.  Training: Adam, one pair per step, on the device.
 
-Judge: from the filtered state of a held-out seed, the coarse solver runs
+Judge: from the filtered state of a held-out seed (a flow of its own:
+its resolved-scale modes differ from every training and calibration
+seed's), the coarse solver runs
 eval-steps steps with and without the network (the plain compact filter is
 the fallback); both are compared with the filtered fine run over the same
 time in kinetic energy (every step) and in the kinetic-energy spectrum at
@@ -38,9 +42,15 @@ def log(*a):
     print("[closure]", *a, file=sys.stderr, flush=True)
 
 
-def tgv(n, seed, mach=0.1, re=1600.0):
+MODES = 0.05   # relative amplitude of the resolved-scale modes that make each seed's flow its own
+
+
+def tgv(n, seed, mach=0.1, re=1600.0, modes=None):
+    """Taylor-Green run of seed: the 1e-3 grid-scale perturbation and
+    resolved-scale Fourier modes (relative amplitude modes) drawn from it."""
+    modes = MODES if modes is None else modes
     xyz = cases.gridtoy(n, n, n, doml0=0.0, doml1=2 * np.pi, domh=np.pi, span=2 * np.pi)
-    q = cases.tgv_state(xyz, (n + 1,) * 3, mach=mach, pert=1e-3, seed=seed)
+    q = cases.tgv_state(xyz, (n + 1,) * 3, mach=mach, pert=1e-3, seed=seed, modes=modes, mode_seed=seed)
     c = Case([cases.periodic_block(n)], dict(fltk=0.49, fltrbc=0.9, reoo=re / mach, tempoo=300.0, cfl=0.8,
                                              viscous=1))
     c.set_grids([xyz])
@@ -105,11 +115,15 @@ def main(argv=None):
     ap.add_argument("--eval-steps", type=int, default=300)
     ap.add_argument("--dt", type=float, default=0.01, help="fine time step")
     ap.add_argument("--out", default="closure_out", help="folder for the weights, losses and summary")
+    ap.add_argument("--modes", type=float, default=MODES,
+                    help="relative amplitude of the resolved-scale modes of each seed's initial flow")
     ap.add_argument("--alphas", default="0.1,0.25,0.5,1",
                     help="comma-separated scales of the network's correction tried on a calibration run (seed "
                          "998, its own fine truth); the judge uses the one with the lowest kinetic-energy error")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
+    global MODES
+    MODES = a.modes
     nf, n = a.fine, a.coarse
     r = nf // n
     dtf, dtc = a.dt, a.dt * r
@@ -221,14 +235,18 @@ def main(argv=None):
             "net": float(np.linalg.norm(sn - st) / np.linalg.norm(st))}
     keep = err["net"] < err["plain"] and serr["net"] < serr["plain"]
     summary = dict(fine=nf, coarse=n, runs=a.runs, pairs=npairs, dt_fine=dtf, dt_coarse=dtc, iters=a.iters,
-                   lr=a.lr, valid_before=before, history=hist, eval_steps=a.eval_steps, alpha=alpha,
+                   lr=a.lr, modes=a.modes, valid_before=before, history=hist, eval_steps=a.eval_steps, alpha=alpha,
                    calibration=calib,
                    t_end=float(fine.clock()["timo"]), ke_rms_rel_err=err, spectrum_rel_err=serr,
                    verdict="network kept" if keep else "fallback: plain compact filter",
                    ke_truth=ke["truth"][::10], ke_plain=ke["plain"][::10], ke_net=ke["net"][::10],
                    spectrum_truth=st.tolist(), spectrum_plain=sp.tolist(), spectrum_net=sn.tolist(),
                    seconds=time.perf_counter() - t0)
-    np.save(os.path.join(a.out, "params.npy"), net.params())
+    # everything needed to rebuild the judged network:
+    #   z = np.load("closure.npz"); net = Net(n); net.set_params(z["params"])
+    #   net.set_norm(z["in_mean"], z["in_scale"], z["alpha"] * z["out_scale"])
+    np.savez(os.path.join(a.out, "closure.npz"), params=net.params(), in_mean=in_mean, in_scale=in_scale,
+             out_scale=out_scale, alpha=np.float64(alpha), coarse=np.int64(n))
     with open(os.path.join(a.out, "summary.json"), "w") as f:
         json.dump(summary, f)
     log(f"KE rms rel err plain {err['plain']:.3e} net {err['net']:.3e}; spectrum rel err plain "
@@ -237,6 +255,7 @@ def main(argv=None):
                                              "spectrum_rel_err", "verdict", "valid_before", "seconds")}))
     for c in (fine, plain, withnet, coarse, target, net):
         c.close()
+    return summary
 
 
 if __name__ == "__main__":

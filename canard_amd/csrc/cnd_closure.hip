@@ -367,6 +367,10 @@ __global__ void k_fill(float *a, float v, int64_t n)
 struct cnd_net {
     int n = 0, V = 0, cap = 0, npairs = 0;
     hipStream_t st = nullptr;
+    // recorded after every use of the network's activation buffers, on
+    // whichever stream ran it; the next use waits for it first, so calls on
+    // different streams (one per context) never overlap on those buffers
+    hipEvent_t done = nullptr;
     Norm nm{};
     float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;   // parameters, gradient, Adam moments
     float *gacc = nullptr;                                            // gw1 | gw2 | gw3 | gb (3 x 32)
@@ -457,6 +461,20 @@ int nalloc(cnd_net *t, T **p, int64_t n)
     return CND_OK;
 }
 
+// Order work on st after every earlier use of the network's buffers, and
+// mark the end of this use.
+int net_begin(cnd_net *t, hipStream_t st)
+{
+    NHIP(t, hipStreamWaitEvent(st, t->done, 0));
+    return CND_OK;
+}
+int net_end(cnd_net *t, hipStream_t st)
+{
+    NHIP(t, hipGetLastError());
+    NHIP(t, hipEventRecord(t->done, st));
+    return CND_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -481,8 +499,9 @@ int cnd_net_create(int n, int capacity, unsigned long long seed, cnd_net **out)
     int r = CND_OK;
     auto ok = [&](int rc) { if (rc) r = rc; return rc == CND_OK; };
     const int64_t S8 = int64_t(kCP) * kTaps * kC0P, S32 = int64_t(kCP) * kTaps * kCP;
-    bool good = hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) == hipSuccess;
-    if (!good) r = nfail(t, CND_EDEVICE, "stream creation failed");
+    bool good = hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) == hipSuccess &&
+                hipEventCreateWithFlags(&t->done, hipEventDisableTiming) == hipSuccess;
+    if (!good) r = nfail(t, CND_EDEVICE, "stream or event creation failed");
     good = good && ok(nalloc(t, &t->p, kNParam)) && ok(nalloc(t, &t->g, kNParam)) && ok(nalloc(t, &t->m1, kNParam)) &&
            ok(nalloc(t, &t->m2, kNParam)) && ok(nalloc(t, &t->gacc, kGacc)) && ok(nalloc(t, &t->W.wf1, S8)) &&
            ok(nalloc(t, &t->W.wf2, S32)) && ok(nalloc(t, &t->W.wf3, S32)) && ok(nalloc(t, &t->W.wd1, S32)) &&
@@ -513,7 +532,7 @@ int cnd_net_create(int n, int capacity, unsigned long long seed, cnd_net **out)
     if (hipMemcpy(t->p, h.data(), sizeof(float) * kNParam, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(t->m1, 0, sizeof(float) * kNParam) != hipSuccess ||
         hipMemset(t->m2, 0, sizeof(float) * kNParam) != hipSuccess || prep(t, t->st) != CND_OK ||
-        hipStreamSynchronize(t->st) != hipSuccess) {
+        hipEventRecord(t->done, t->st) != hipSuccess || hipStreamSynchronize(t->st) != hipSuccess) {
         cnd_net_destroy(t);
         return nfail(nullptr, CND_EDEVICE, "initial upload failed");
     }
@@ -524,11 +543,13 @@ int cnd_net_create(int n, int capacity, unsigned long long seed, cnd_net **out)
 void cnd_net_destroy(cnd_net *t)
 {
     if (!t) return;
+    if (t->done) (void)hipEventSynchronize(t->done);
     if (t->st) (void)hipStreamSynchronize(t->st);
     void *ptrs[] = {t->p, t->g, t->m1, t->m2, t->gacc, t->W.wf1, t->W.wf2, t->W.wf3, t->W.wd1, t->W.wd2, t->W.wd3,
                     t->W.bb1, t->W.bb2, t->W.bb3, t->x0, t->a1, t->a2, t->g3, t->g2, t->g1, t->y, t->gx, t->hx,
                     t->loss, t->din, t->dtg};
     for (void *p : ptrs) if (p) (void)hipFree(p);
+    if (t->done) (void)hipEventDestroy(t->done);
     if (t->st) (void)hipStreamDestroy(t->st);
     delete t;
 }
@@ -544,10 +565,11 @@ int cnd_net_get_params(cnd_net *t, float *w)
 int cnd_net_set_params(cnd_net *t, const float *w)
 {
     if (!t || !w) return nfail(t, CND_EINVAL, "cnd_net_set_params: bad argument");
+    NHIP(t, hipEventSynchronize(t->done));
     NHIP(t, hipStreamSynchronize(t->st));
     NHIP(t, hipMemcpy(t->p, w, sizeof(float) * kNParam, hipMemcpyHostToDevice));
     int r;
-    if ((r = prep(t, t->st))) return r;
+    if ((r = prep(t, t->st)) || (r = net_end(t, t->st))) return r;
     NHIP(t, hipStreamSynchronize(t->st));
     return CND_OK;
 }
@@ -597,12 +619,13 @@ int cnd_net_add_pair(cnd_net *t, cnd_ctx *coarse, int bc, cnd_ctx *target, int b
     if ((r = coarse_view(t, coarse, bc, C, "cnd_net_add_pair"))) return r;
     if ((r = coarse_view(t, target, bt, T, "cnd_net_add_pair (target)"))) return r;
     NHIP(t, hipStreamSynchronize(T.st));
+    if ((r = net_begin(t, C.st))) return r;
     const int64_t V = t->V;
     hipLaunchKernelGGL(k_pack_state, dim3(blocks_for(V, 256)), dim3(256), 0, C.st, C.q, C.NT, C.off, t->n, t->nm,
                        t->din + int64_t(t->npairs) * V * kC0P);
     hipLaunchKernelGGL(k_target, dim3(blocks_for(V, 256)), dim3(256), 0, C.st, C.q, C.NT, C.off, T.q, T.NT, T.off,
                        t->n, t->nm, t->dtg + int64_t(t->npairs) * V * kC3);
-    NHIP(t, hipGetLastError());
+    if ((r = net_end(t, C.st))) return r;
     NHIP(t, hipStreamSynchronize(C.st));
     return t->npairs++;
 }
@@ -613,10 +636,13 @@ int cnd_net_add_pair_host(cnd_net *t, const float *x, const float *tg)
     if (t->npairs >= t->cap) return nfail(t, CND_EINVAL, "cnd_net_add_pair_host: the pair store is full");
     const int64_t V = t->V;
     hipStream_t st = t->st;
+    int r;
+    if ((r = net_begin(t, st))) return r;
     NHIP(t, hipMemcpyAsync(t->hx, x, sizeof(float) * V * kC0, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_pack_host_in, dim3(blocks_for(V, 256)), dim3(256), 0, st, t->hx,
                        t->din + int64_t(t->npairs) * V * kC0P, int(V));
     NHIP(t, hipMemcpyAsync(t->dtg + int64_t(t->npairs) * V * kC3, tg, sizeof(float) * V * kC3, hipMemcpyHostToDevice, st));
+    if ((r = net_end(t, st))) return r;
     NHIP(t, hipStreamSynchronize(st));
     return t->npairs++;
 }
@@ -631,8 +657,9 @@ int cnd_net_train(cnd_net *t, int iters, double lr, unsigned long long seed, dou
     std::uniform_int_distribution<int> pick(0, t->npairs - 1);
     const int64_t V = t->V;
     hipStream_t st = t->st;
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, reinterpret_cast<float *>(t->loss), 0.f, 2);
     int r;
+    if ((r = net_begin(t, st))) return r;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, reinterpret_cast<float *>(t->loss), 0.f, 2);
     for (int it = 0; it < iters; ++it) {
         const int k = pick(rng);
         const bf16 *x0 = t->din + int64_t(k) * V * kC0P;
@@ -648,6 +675,7 @@ int cnd_net_train(cnd_net *t, int iters, double lr, unsigned long long seed, dou
     }
     double s = 0.0;
     NHIP(t, hipMemcpyAsync(&s, t->loss, sizeof(double), hipMemcpyDeviceToHost, st));
+    if ((r = net_end(t, st))) return r;
     NHIP(t, hipStreamSynchronize(st));
     if (mean_loss) *mean_loss = iters > 0 ? s / (double(iters) * 5.0 * double(V)) : 0.0;
     return CND_OK;
@@ -659,8 +687,9 @@ int cnd_net_eval(cnd_net *t, int first, int count, double *mean_loss)
         return nfail(t, CND_EINVAL, "cnd_net_eval: bad range");
     const int64_t V = t->V;
     hipStream_t st = t->st;
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, reinterpret_cast<float *>(t->loss), 0.f, 2);
     int r;
+    if ((r = net_begin(t, st))) return r;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, reinterpret_cast<float *>(t->loss), 0.f, 2);
     for (int k = first; k < first + count; ++k) {
         if ((r = forward(t, t->din + int64_t(k) * V * kC0P, st))) return r;
         hipLaunchKernelGGL(k_mse, dim3(blocks_for(V, 256)), dim3(256), 0, st, t->y, t->dtg + int64_t(k) * V * kC3,
@@ -668,6 +697,7 @@ int cnd_net_eval(cnd_net *t, int first, int count, double *mean_loss)
     }
     double s = 0.0;
     NHIP(t, hipMemcpyAsync(&s, t->loss, sizeof(double), hipMemcpyDeviceToHost, st));
+    if ((r = net_end(t, st))) return r;
     NHIP(t, hipStreamSynchronize(st));
     *mean_loss = s / (double(count) * 5.0 * double(V));
     return CND_OK;
@@ -679,15 +709,14 @@ int cnd_net_apply(cnd_net *t, cnd_ctx *ctx, int b)
     CtxView C;
     int r;
     if ((r = coarse_view(t, ctx, b, C, "cnd_net_apply"))) return r;
-    NHIP(t, hipStreamSynchronize(t->st));
+    if ((r = net_begin(t, C.st))) return r;
     const int64_t V = t->V;
     hipLaunchKernelGGL(k_pack_state, dim3(blocks_for(V, 256)), dim3(256), 0, C.st, C.q, C.NT, C.off, t->n, t->nm, t->x0);
     if ((r = forward(t, t->x0, C.st))) return r;
     const int64_t P = int64_t(t->n + 1) * (t->n + 1) * (t->n + 1);
     hipLaunchKernelGGL(k_add_correction, dim3(blocks_for(P, 256)), dim3(256), 0, C.st, C.q, C.NT, C.off, t->n, t->nm,
                        t->y);
-    NHIP(t, hipGetLastError());
-    return CND_OK;
+    return net_end(t, C.st);
 }
 
 int cnd_net_forward_host(cnd_net *t, const float *x, float *y)
@@ -695,11 +724,13 @@ int cnd_net_forward_host(cnd_net *t, const float *x, float *y)
     if (!t || !x || !y) return nfail(t, CND_EINVAL, "cnd_net_forward_host: bad argument");
     const int64_t V = t->V;
     hipStream_t st = t->st;
+    int r;
+    if ((r = net_begin(t, st))) return r;
     NHIP(t, hipMemcpyAsync(t->hx, x, sizeof(float) * V * kC0, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_pack_host_in, dim3(blocks_for(V, 256)), dim3(256), 0, st, t->hx, t->x0, int(V));
-    int r;
     if ((r = forward(t, t->x0, st))) return r;
     NHIP(t, hipMemcpyAsync(y, t->y, sizeof(float) * V * kC3, hipMemcpyDeviceToHost, st));
+    if ((r = net_end(t, st))) return r;
     NHIP(t, hipStreamSynchronize(st));
     return CND_OK;
 }
@@ -709,15 +740,17 @@ int cnd_net_grad_host(cnd_net *t, const float *x, const float *gy, float *gx, fl
     if (!t || !x || !gy || !gx || !gp) return nfail(t, CND_EINVAL, "cnd_net_grad_host: bad argument");
     const int64_t V = t->V;
     hipStream_t st = t->st;
+    int r;
+    if ((r = net_begin(t, st))) return r;
     NHIP(t, hipMemcpyAsync(t->hx, x, sizeof(float) * V * kC0, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_pack_host_in, dim3(blocks_for(V, 256)), dim3(256), 0, st, t->hx, t->x0, int(V));
-    int r;
     if ((r = forward(t, t->x0, st))) return r;
     NHIP(t, hipMemcpyAsync(t->gx, gy, sizeof(float) * V * kC3, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_pad_grad, dim3(blocks_for(V, 256)), dim3(256), 0, st, t->gx, t->g3, int(V));
     if ((r = backward(t, t->x0, t->hx, st))) return r;
     NHIP(t, hipMemcpyAsync(gx, t->hx, sizeof(float) * V * kC0, hipMemcpyDeviceToHost, st));
     NHIP(t, hipMemcpyAsync(gp, t->g, sizeof(float) * kNParam, hipMemcpyDeviceToHost, st));
+    if ((r = net_end(t, st))) return r;
     NHIP(t, hipStreamSynchronize(st));
     return CND_OK;
 }

@@ -25,6 +25,8 @@ constexpr int64_t kAlignPts = 256;
 // Internal header: every translation unit of libcnd works in namespace cnd.
 using namespace cnd;
 
+struct OutputState;   // output buffers and writer thread (cnd_output.hip)
+
 // ==== context ===================================================================
 struct JuncDev {
     int n = 0;
@@ -79,10 +81,17 @@ struct cnd_ctx {
     // mixed precision: FP32 gradients and stresses; FP32 LU / homogeneous
     // tables and the FP64 banded LHS for the refinement
     float *gradf = nullptr, *dlu32 = nullptr, *dseg32[kNumL] = {};
+    // mixed precision: the 15 stored fluxes (less the instance's offsets)
+    // and the three right-hand-side buffers in FP32
+    float *fl32 = nullptr, *de32 = nullptr;
     double *dlhs = nullptr;
     JuncDev jc, je;
     std::vector<char> grid_set;
     bool metrics_ready = false, state_in_qo = false;
+    OutputState *out = nullptr;   // created by the first output call
+    // the call the stage-by-stage API expects next: 0 a new step (cnd_filter
+    // or cnd_step), 1..4 cnd_rk_stage(nk), 5 cnd_end_step
+    int next_call = 0;
     bool any_gcbc = false, any_wall = false, any_avg = false, any_sponge = false, any_visc = false;
     bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
@@ -222,5 +231,8 @@ int run_flux(cnd_ctx *c, int d, const double *cur);
 // zeta flux sweep of the stored fluxes fused with the RK update of stage nk,
 // written to qa.
 int run_flux_update(cnd_ctx *c, int nk, double *qa);
+
+// Joins an output write in flight and frees the output buffers (cnd_destroy).
+void output_release(cnd_ctx *c);
 
 }  // namespace cnd

@@ -40,13 +40,14 @@ namespace {
 
 // Viscous stresses and heat flux (src/mo_physics.f90:245-299) and, from
 // them, the fluxes of all three directions (src/mo_physics.f90:319-382),
-// stored as F[5 d + m] for the flux sweeps.  TG: storage type of the
-// gradients (float in mixed mode); stresses and fluxes are FP64.
-template <class TG>
+// stored as F[5 d + m] for the flux sweeps.  TS: storage type of the
+// gradients and fluxes (float in mixed mode: the fluxes are then stored less
+// the instance's offsets foff); stresses and fluxes are computed in FP64.
+template <class TS>
 __global__ __launch_bounds__(kPts) void k_visc_flux(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
                                                    const Tile *__restrict__ tiles, const double *__restrict__ q,
-                                                   const TG *__restrict__ grad, const double *__restrict__ met,
-                                                   double *__restrict__ F, int64_t NT)
+                                                   const TS *__restrict__ grad, const double *__restrict__ met,
+                                                   TS *__restrict__ F, int64_t NT)
 {
     POINT_PROLOGUE
     const InstDev &I = inst[B.inst];
@@ -82,7 +83,10 @@ __global__ __launch_bounds__(kPts) void k_visc_flux(const BlkDev *__restrict__ b
         double o[5];
         dir_flux<true>(q1, q2, q3, q4, q5, u, v, w, p, m, I.umf, ts, o);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) F[(5 * d + c) * NT + g] = o[c];
+        for (int c = 0; c < 5; ++c) {
+            if constexpr (std::is_same<TS, float>::value) F[(5 * d + c) * NT + g] = float(o[c] - I.foff[d][c]);
+            else F[(5 * d + c) * NT + g] = o[c];
+        }
     }
 }
 
@@ -181,8 +185,11 @@ __device__ void movef(InstDev &I, double dtko, double dtk)
 }
 
 // Start of stage nk: frame velocity for fluxes (mo_canard_driver.f90:261-263)
-// and, in stage 1, clearing of the time-step maxima.
-__global__ void k_stage_begin(InstDev *inst, int ninst, int nk)
+// and, in stage 1, clearing of the time-step maxima.  In mixed precision
+// also the flux offsets: the inviscid fluxes of the stage's state q at the
+// instance's first point.
+__global__ void k_stage_begin(InstDev *inst, int ninst, int nk, const double *q, const double *met, int64_t NT,
+                              int mixed)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
@@ -190,6 +197,17 @@ __global__ void k_stage_begin(InstDev *inst, int ninst, int nk)
     const int a = nk - 2 > 0 ? (nk - 2 < 1 ? nk - 2 : 1) : 0, c = nk - 1 < 1 ? nk - 1 : 1;
     movef(I, (I.dt * double(a)) / double(4 - nk + 3), (I.dt * double(c)) / double(4 - nk + 2));
     if (nk == 1) { I.red[0] = 0ull; I.red[1] = 0ull; }
+    if (mixed) {
+        const int64_t g = I.g0;
+        const double q1 = q[g], q2 = q[NT + g], q3 = q[2 * NT + g], q4 = q[3 * NT + g], q5 = q[4 * NT + g];
+        const double ri = 1.0 / q1;
+        const double u = q2 * ri, v = q3 * ri, w = q4 * ri;
+        const double p = kGm1 * (q5 - 0.5 * ((q2 * u + q3 * v) + q4 * w));
+        for (int d = 0; d < 3; ++d) {
+            const double m[3] = {met[(3 * d) * NT + g], met[(3 * d + 1) * NT + g], met[(3 * d + 2) * NT + g]};
+            dir_flux<false>(q1, q2, q3, q4, q5, u, v, w, p, m, I.umf, nullptr, I.foff[d]);
+        }
+    }
 }
 
 // dt schedule (mo_canard_driver.f90:278-285).
@@ -211,6 +229,15 @@ __global__ void k_sched(InstDev *inst, const TabDev *tabs, int ninst)
     if (k >= 0 && k < 64) r = tabs[i].ramp[k];
     else { const double s = sin((0.05 * 3.141592653589793) * double(k)); r = s * s; }
     I.dt = I.dts + (I.dte - I.dts) * r;
+    // output time (mo_canard_driver.f90:286-291)
+    I.nout = 0;
+    if (I.ndata > 0) {
+        const double res = (double(I.ndati + 1) * I.tmax) / double(I.ndata);
+        if ((I.timo - res) * ((I.timo + I.dt) - res) <= 0.0) {
+            I.nout = 1;
+            I.ndati = I.ndati + 1;
+        }
+    }
 }
 
 // frame velocity for the update half of the stage (mo_canard_driver.f90:325-327)
@@ -251,42 +278,13 @@ __global__ void k_record_clock(InstDev *inst, int ninst, int reset)
     inst[i].dtsum = reset ? 0.0 : inst[i].dtsum + inst[i].dt;
 }
 
-// Output variables of one block (mo_canard_driver.f90:379-400): the time
-// average qb / dtsum (the filtered state qo after the first step, n = 1)
-// as density, velocity in the fixed frame and pressure, in FP32, stored
-// variable after variable over the block's points.
-__global__ __launch_bounds__(kPts) void k_record_out(const BlkDev *__restrict__ blks,
-                                                    const InstDev *__restrict__ inst, int b,
-                                                    const double *__restrict__ qo, const double *__restrict__ qb,
-                                                    float *__restrict__ out, int64_t NT)
-{
-    const BlkDev &B = blks[b];
-    const InstDev &I = inst[B.inst];
-    const int64_t npt = int64_t(B.n[0]) * B.n[1] * B.n[2];
-    const int64_t lp = int64_t(blockIdx.x) * kPts + threadIdx.x;
-    if (lp >= npt) return;
-    const int64_t g = B.off + lp;
-    double q[5];
-    if (I.n == 1) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) q[m] = qo[m * NT + g];
-    } else {
-        const double ra0 = 1.0 / I.dtsum;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) q[m] = ra0 * qb[m * NT + g];
-    }
-    const double rr = 1.0 / q[0];
-    out[lp] = float(q[0]);
-#pragma unroll
-    for (int m = 1; m < 4; ++m) out[m * npt + lp] = float(rr * q[m] + I.umf[m - 1]);
-    out[4 * npt + lp] = float(kGm1 * (q[4] - (0.5 * rr) * ((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3])));
-}
-
 // Sponge source and RK update (mo_sponge.f90:75-88,
-// mo_canard_driver.f90:329-334): qa = qo - dt/(5-nk) * yaco * de.
+// mo_canard_driver.f90:329-334): qa = qo - dt/(5-nk) * yaco * de.  TD:
+// storage type of the right-hand-side buffers (float in mixed mode).
+template <class TD>
 __global__ __launch_bounds__(kPts) void k_update(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
                                                 const Tile *__restrict__ tiles, const double *cur,
-                                                const double *__restrict__ qo, double *__restrict__ de,
+                                                const double *__restrict__ qo, const TD *__restrict__ de,
                                                 const double *__restrict__ met, const double *__restrict__ sponge,
                                                 double *qa, int64_t NT, int nk)
 {
@@ -296,7 +294,8 @@ __global__ __launch_bounds__(kPts) void k_update(const BlkDev *__restrict__ blks
     const double r = dtk * met[9 * NT + g];
     double d[5];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) d[m] = (de[m * NT + g] + de[(5 + m) * NT + g]) + de[(10 + m) * NT + g];
+    for (int m = 0; m < 5; ++m)
+        d[m] = (double(de[m * NT + g]) + double(de[(5 + m) * NT + g])) + double(de[(10 + m) * NT + g]);
     if (I.sponge) {
         const double a = sponge[g], bb = sponge[NT + g];
         if (a != 0.0) {
@@ -379,10 +378,12 @@ __device__ __forceinline__ double pressure(const double *q, int64_t NT, int64_t 
 
 // gcbc_setup: characteristic transform of the face derivative (BC 10) or of
 // the full right-hand side (walls, curvilinear interfaces).  grid (tiles, side).
+// TD: storage type of the right-hand-side buffers.
+template <class TD>
 __global__ __launch_bounds__(kLine) void k_gcbc_setup(const BlkDev *__restrict__ blks,
                                                      const InstDev *__restrict__ inst,
                                                      const Tile *__restrict__ tiles, int d, const double *q,
-                                                     const double *de, const double *met, const double *cmf,
+                                                     const TD *de, const double *met, const double *cmf,
                                                      double *drva, int64_t NT, int64_t FT)
 {
     const Tile t = tiles[blockIdx.x];
@@ -399,7 +400,8 @@ __global__ __launch_bounds__(kLine) void k_gcbc_setup(const BlkDev *__restrict__
     const Elem e = eleme(cm, q[g], q24, pressure(q, NT, g), inst[B.inst].umf);
     xtq2r(e, cm, xt);
     for (int c = 0; c < 5; ++c)
-        cha[c] = ra0 * drva[(side * 5 + c) * FT + fi] + ra1 * ((de[c * NT + g] + de[(5 + c) * NT + g]) + de[(10 + c) * NT + g]);
+        cha[c] = ra0 * drva[(side * 5 + c) * FT + fi] +
+                 ra1 * ((double(de[c * NT + g]) + double(de[(5 + c) * NT + g])) + double(de[(10 + c) * NT + g]));
     const double y = met[9 * NT + g];
     for (int c = 0; c < 5; ++c) yc[c] = y * cha[c];
     for (int r = 0; r < 5; ++r) {
@@ -410,11 +412,12 @@ __global__ __launch_bounds__(kLine) void k_gcbc_setup(const BlkDev *__restrict__
 }
 
 // gcbc_update of face (d, side); faces are applied in the reference's order.
+template <class TD>
 __global__ __launch_bounds__(kLine) void k_gcbc_update(const BlkDev *__restrict__ blks,
                                                       const InstDev *__restrict__ inst,
                                                       const TabDev *__restrict__ tabs,
                                                       const Tile *__restrict__ tiles, int d, int side,
-                                                      const double *q, double *de, const double *met,
+                                                      const double *q, TD *de, const double *met,
                                                       const double *cmf, const double *drva, int64_t NT, int64_t FT)
 {
     const Tile t = tiles[blockIdx.x];
@@ -482,10 +485,10 @@ __global__ __launch_bounds__(kLine) void k_gcbc_update(const BlkDev *__restrict_
         // the right-hand side is the sum of the three direction buffers: the
         // corrected sum goes to the first, the others are cleared
         for (int c = 0; c < 5; ++c) {
-            const double tot = (de[c * NT + gl] + de[(5 + c) * NT + gl]) + de[(10 + c) * NT + gl];
-            de[c * NT + gl] = tot + sbcc * dha[c];
-            de[(5 + c) * NT + gl] = 0.0;
-            de[(10 + c) * NT + gl] = 0.0;
+            const double tot = (double(de[c * NT + gl]) + double(de[(5 + c) * NT + gl])) + double(de[(10 + c) * NT + gl]);
+            de[c * NT + gl] = TD(tot + sbcc * dha[c]);
+            de[(5 + c) * NT + gl] = TD(0);
+            de[(10 + c) * NT + gl] = TD(0);
         }
     }
 }
@@ -693,6 +696,11 @@ int upload_inst(cnd_ctx *c)
         I.srefp1dre = (I.srefoo + 1.0) / I.reoo;
         I.uoo[0] = p.amach1; I.uoo[1] = p.amach2; I.uoo[2] = p.amach3;
         I.timf = p.timf; I.cfl = p.cfl; I.nsmf = p.nsmf; I.viscous = p.viscous; I.sponge = p.sponge;
+        I.ndata = 0;
+        I.ndati = -1;
+        I.g0 = -1;
+        for (int b = 0; b < c->nblk && I.g0 < 0; ++b)
+            if (c->spec[b].inst == i) I.g0 = c->off[b];
     }
     HIPX(c, hipMemcpy(c->dinst, h.data(), sizeof(InstDev) * c->ninst, hipMemcpyHostToDevice));
     return CND_OK;
@@ -777,7 +785,9 @@ int enqueue_stage(cnd_ctx *c, int nk)
 {
     const double *cur = nk == 1 ? c->qo : c->qa;
     const unsigned ib = unsigned((c->ninst + 63) / 64);
-    hipLaunchKernelGGL(k_stage_begin, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, nk);
+    const bool mix = c->precision == CND_MIXED;
+    hipLaunchKernelGGL(k_stage_begin, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, nk, cur, c->met, c->NT,
+                       int(mix));
     if (nk == 1) {
         Section sec(c, CND_SEC_TSTEP, 1);
         hipLaunchKernelGGL(k_tstep, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->met,
@@ -794,14 +804,14 @@ int enqueue_stage(cnd_ctx *c, int nk)
     }
     if (c->any_visc) {
         Section sec(c, CND_SEC_STRESS, 1);
-        if (c->precision == CND_MIXED)
+        if (mix)
             hipLaunchKernelGGL(k_visc_flux<float>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
-                               c->gradf, c->met, c->fl, c->NT);
+                               c->gradf, c->met, c->fl32, c->NT);
         else
             hipLaunchKernelGGL(k_visc_flux<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt,
                                cur, c->grad, c->met, c->fl, c->NT);
     }
-    const bool fused = fuse_possible(c) && (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, 2));
+    const bool fused = fuse_possible(c) && (mix || seg_on(c, CND_KIND_FLUX, 2));
     {
     Section sec(c, CND_SEC_FLUX, fused ? 2 : 3);
     for (int d = 0; d < (fused ? 2 : 3); ++d)
@@ -809,22 +819,38 @@ int enqueue_stage(cnd_ctx *c, int nk)
     }
     if (c->any_gcbc) {
         Section sec(c, CND_SEC_GCBC, 9);
-        for (int d = 0; d < 3; ++d)
-            hipLaunchKernelGGL(k_gcbc_setup, tiles_grid(c, d, 2), dim3(kLine), 0, c->st, c->dblk, c->dinst,
-                               c->dtiles[d], d, cur, c->de, c->met, c->cmf[d], c->drva[d], c->NT, c->FT[d]);
-        for (int d = 0; d < 3; ++d)
-            for (int s = 0; s < 2; ++s)
-                hipLaunchKernelGGL(k_gcbc_update, tiles_grid(c, d, 1), dim3(kLine), 0, c->st, c->dblk, c->dinst,
-                                   c->dtab, c->dtiles[d], d, s, cur, c->de, c->met, c->cmf[d], c->drva[d], c->NT,
+        for (int d = 0; d < 3; ++d) {
+            if (mix)
+                hipLaunchKernelGGL(k_gcbc_setup<float>, tiles_grid(c, d, 2), dim3(kLine), 0, c->st, c->dblk, c->dinst,
+                                   c->dtiles[d], d, cur, c->de32, c->met, c->cmf[d], c->drva[d], c->NT, c->FT[d]);
+            else
+                hipLaunchKernelGGL(k_gcbc_setup<double>, tiles_grid(c, d, 2), dim3(kLine), 0, c->st, c->dblk,
+                                   c->dinst, c->dtiles[d], d, cur, c->de, c->met, c->cmf[d], c->drva[d], c->NT,
                                    c->FT[d]);
+        }
+        for (int d = 0; d < 3; ++d)
+            for (int s = 0; s < 2; ++s) {
+                if (mix)
+                    hipLaunchKernelGGL(k_gcbc_update<float>, tiles_grid(c, d, 1), dim3(kLine), 0, c->st, c->dblk,
+                                       c->dinst, c->dtab, c->dtiles[d], d, s, cur, c->de32, c->met, c->cmf[d],
+                                       c->drva[d], c->NT, c->FT[d]);
+                else
+                    hipLaunchKernelGGL(k_gcbc_update<double>, tiles_grid(c, d, 1), dim3(kLine), 0, c->st, c->dblk,
+                                       c->dinst, c->dtab, c->dtiles[d], d, s, cur, c->de, c->met, c->cmf[d],
+                                       c->drva[d], c->NT, c->FT[d]);
+            }
     }
     if (fused) {
         Section sec(c, CND_SEC_UPDATE, 1);
         if ((r = run_flux_update(c, nk, c->qa))) return r;
     } else {
         Section sec(c, CND_SEC_UPDATE, 1);
-        hipLaunchKernelGGL(k_update, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->qo, c->de,
-                           c->met, c->sponge, c->qa, c->NT, nk);
+        if (mix)
+            hipLaunchKernelGGL(k_update<float>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
+                               c->qo, c->de32, c->met, c->sponge, c->qa, c->NT, nk);
+        else
+            hipLaunchKernelGGL(k_update<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
+                               c->qo, c->de, c->met, c->sponge, c->qa, c->NT, nk);
     }
     hipLaunchKernelGGL(k_stage_mid, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, nk);
     Section avg(c, CND_SEC_AVG, 1);
@@ -1048,7 +1074,7 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     for (int li = 0; good && li < kNumL; ++li) good = ok(dalloc(c, &c->dseg[li], segtab[li].size()));
     if (good && c->any_sponge) good = ok(dalloc(c, &c->sponge, 2 * NT));
     if (good && precision == CND_MIXED)
-        good = ok(dalloc(c, &c->gradf, 12 * NT)) &&
+        good = ok(dalloc(c, &c->gradf, 12 * NT)) && ok(dalloc(c, &c->fl32, 15 * NT)) && ok(dalloc(c, &c->de32, 15 * NT)) &&
                ok(dalloc(c, &c->dlu32, lu.size())) && ok(dalloc(c, &c->dlhs, lhs.size()));
     for (int li = 0; good && precision == CND_MIXED && li < kNumL; ++li)
         good = ok(dalloc(c, &c->dseg32[li], segtab[li].size()));
@@ -1104,6 +1130,8 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     if (!r) {
         hipError_t e = hipMemset(c->de, 0, sizeof(double) * 15 * NT);
         if (e == hipSuccess) e = hipMemset(c->fl, 0, sizeof(double) * 15 * NT);
+        if (e == hipSuccess && c->de32) e = hipMemset(c->de32, 0, sizeof(float) * 15 * NT);
+        if (e == hipSuccess && c->fl32) e = hipMemset(c->fl32, 0, sizeof(float) * 15 * NT);
         if (e != hipSuccess) r = fail(c, CND_EDEVICE, "memset");
     }
     // junction plans (search_point / search_line)
@@ -1135,11 +1163,12 @@ void cnd_destroy(cnd_ctx *c)
 {
     if (!c) return;
     if (c->st) hipStreamSynchronize(c->st);
+    output_release(c);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->fl, c->met, c->ys,
-                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->dlu32, c->dlhs, c->jc.tgt, c->jc.src,
+                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->fl32, c->de32, c->dlu32, c->dlhs, c->jc.tgt, c->jc.src,
                     c->jc.first, c->jc.scale, c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp,
                     c->qb};
     for (void *p : ptrs) if (p) hipFree(p);
@@ -1336,7 +1365,7 @@ int cnd_tune(cnd_ctx *c)
 {
     int r;
     if ((r = ready(c))) return r;
-    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_tune: a step is half done");
+    if (c->next_call != 0) return fail(c, CND_ESTATE, "cnd_tune: a step is half done");
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     c->tuned = false;
     return tune(c);
@@ -1357,7 +1386,7 @@ int cnd_set_shape(cnd_ctx *c, int kind, int dir, int shape)
 {
     if (!c || kind < 0 || kind >= CND_NKIND || dir < 0 || dir > 2 || shape < -1 || shape >= kNumShapes)
         return fail(c, CND_EINVAL, "cnd_set_shape: bad argument");
-    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_set_shape: a step is half done");
+    if (c->next_call != 0) return fail(c, CND_ESTATE, "cnd_set_shape: a step is half done");
     if (shape == kTiled && c->precision == CND_MIXED)
         return fail(c, CND_EINVAL, "cnd_set_shape: mixed precision runs on the segmented sweeps only");
     const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir, kind_nv(c, kind));
@@ -1384,14 +1413,25 @@ int cnd_tuning(const cnd_ctx *c, int kind, int dir, int *shape, int *seg_len, in
     return CND_OK;
 }
 
+// What the stage-by-stage API expects next, for refusal messages.
+static std::string expected_call(const cnd_ctx *c)
+{
+    if (c->next_call == 0) return "cnd_filter or cnd_step";
+    if (c->next_call == 5) return "cnd_end_step";
+    return "cnd_rk_stage(" + std::to_string(c->next_call) + ")";
+}
+
 int cnd_filter(cnd_ctx *c)
 {
     int r;
     if ((r = ready(c))) return r;
-    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_filter: the filter phase already ran for this step");
+    if (c->next_call == 1) return fail(c, CND_ESTATE, "cnd_filter: the filter phase already ran for this step");
+    if (c->next_call != 0)
+        return fail(c, CND_ESTATE, "cnd_filter: a step is half done (expected " + expected_call(c) + ")");
     if ((r = tune(c))) return r;
     if ((r = enqueue_filter(c))) return r;
     c->state_in_qo = true;
+    c->next_call = 1;
     return CND_OK;
 }
 
@@ -1400,10 +1440,14 @@ int cnd_rk_stage(cnd_ctx *c, int nk)
     int r;
     if ((r = ready(c))) return r;
     if (nk < 1 || nk > 4) return fail(c, CND_EINVAL, "cnd_rk_stage: nk must be 1..4");
-    if ((nk == 1) != c->state_in_qo) return fail(c, CND_ESTATE, "cnd_rk_stage: stage 1 must follow cnd_filter");
+    if (nk == 1 && c->next_call == 0) return fail(c, CND_ESTATE, "cnd_rk_stage: stage 1 must follow cnd_filter");
+    if (nk != c->next_call)
+        return fail(c, CND_ESTATE, "cnd_rk_stage: stage " + std::to_string(nk) + " out of order (expected " +
+                                       expected_call(c) + ")");
     if ((r = tune(c))) return r;
     if ((r = enqueue_stage(c, nk))) return r;
     c->state_in_qo = false;
+    c->next_call = nk + 1;
     return CND_OK;
 }
 
@@ -1411,7 +1455,12 @@ int cnd_end_step(cnd_ctx *c)
 {
     int r;
     if ((r = ready(c))) return r;
-    return enqueue_end(c);
+    if (c->next_call != 5)
+        return fail(c, CND_ESTATE, "cnd_end_step: the four stages of the step have not run (expected " +
+                                       expected_call(c) + ")");
+    if ((r = enqueue_end(c))) return r;
+    c->next_call = 0;
+    return CND_OK;
 }
 
 int cnd_record_reset(cnd_ctx *c)
@@ -1427,7 +1476,7 @@ int cnd_record_reset(cnd_ctx *c)
 int cnd_record(cnd_ctx *c, int on)
 {
     if (!c) return fail(c, CND_EINVAL, "null context");
-    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_record: a step is half done");
+    if (c->next_call != 0) return fail(c, CND_ESTATE, "cnd_record: a step is half done");
     if (on && !c->qb) {
         int r;
         if ((r = dalloc(c, &c->qb, 5 * c->NT))) return r;
@@ -1435,22 +1484,6 @@ int cnd_record(cnd_ctx *c, int on)
     }
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }   // the step's launches change
     c->recording = on != 0;
-    return CND_OK;
-}
-
-int cnd_record_read(cnd_ctx *c, int b, float *out)
-{
-    if (!c || !out || b < 0 || b >= c->nblk) return fail(c, CND_EINVAL, "cnd_record_read: bad argument");
-    if (!c->qb) return fail(c, CND_ESTATE, "cnd_record_read: recording was never enabled (cnd_record)");
-    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_record_read: a step is half done");
-    const int64_t n = int64_t(c->spec[b].n[0]) * c->spec[b].n[1] * c->spec[b].n[2];
-    float *dev = reinterpret_cast<float *>(c->grad);   // gradient scratch: 12 NT doubles >= 5 n floats
-    hipLaunchKernelGGL(k_record_out, dim3(unsigned((n + kPts - 1) / kPts)), dim3(kPts), 0, c->st, c->dblk, c->dinst, b,
-                       c->qo, c->qb, dev, c->NT);
-    int r;
-    if ((r = check_launch(c))) return r;
-    HIPX(c, hipMemcpyAsync(out, dev, sizeof(float) * 5 * n, hipMemcpyDeviceToHost, c->st));
-    HIPX(c, hipStreamSynchronize(c->st));
     return CND_OK;
 }
 
@@ -1465,7 +1498,8 @@ int cnd_step(cnd_ctx *c, int nsteps)
 {
     int r;
     if ((r = ready(c))) return r;
-    if (c->state_in_qo) return fail(c, CND_ESTATE, "cnd_step: a step is half done (cnd_filter without stages)");
+    if (c->next_call != 0)
+        return fail(c, CND_ESTATE, "cnd_step: a step is half done (expected " + expected_call(c) + ")");
     if (nsteps < 0) return fail(c, CND_EINVAL, "cnd_step: negative step count");
     if ((r = tune(c))) return r;
     if (!c->use_graph) {

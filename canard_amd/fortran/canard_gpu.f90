@@ -17,9 +17,13 @@
 !   tgvmach (0.1)  Taylor-Green Mach number
 !   sponge    (0)  apply the sponge (mo_sponge.f90)
 !   graph     (0)  replay whole steps as a hipGraph
-! Grids: gridtype 0 (gridtoy) and 1 (gridtoy_multi) of mo_gridgen.f90, or
-! gridtype 3: read misc/gridzNNN.dat written by the caller (the record
-! layout of mo_io.f90:151-180, one record of (x,y,z) per point).
+! Grids: gridtype 0 (gridtoy) and 1 (gridtoy_multi) of mo_gridgen.f90,
+! generated on the device, or gridtype 3: read misc/gridzNNN.dat written by
+! the caller (the record layout of mo_io.f90:151-180, one record of (x,y,z)
+! per point) and uploaded.
+! Output (ndata >= 0): data/grid.plt at set-up and data/nNNN.plt at the
+! output times k tmax / ndata, Tecplot files of the recorded averages,
+! written by a host thread while the following steps run.
 
 program canard_gpu
    use iso_c_binding
@@ -63,7 +67,9 @@ program canard_gpu
    type(c_ptr) :: ctx
    real(nr), allocatable :: xyz(:), q(:)
    integer :: mb, nb, npt, n, ndt, nsec, rc
-   real(nr) :: dt, dts, dte, timo, ms
+   integer(c_int) :: due, ndati
+   real(nr) :: dt, dts, dte, timo, ms, tout
+   character(len=32) :: fname
    integer(c_long_long) :: launches
    character(len=8), parameter :: secname(0:7) = [character(len=8) :: 'Filters ', 'TimeStep', 'VSgrad  ', &
                                                    'VSstress', 'Fluxes  ', 'GCBC    ', 'Update  ', 'Average ']
@@ -114,11 +120,21 @@ program canard_gpu
    ! ---- grid and metrics -------------------------------------------------------------
    write(*,*) 'Canard: grid generation'
    do mb = 0, mbk
-      npt = (lximb(mb)+1) * (letmb(mb)+1) * (lzemb(mb)+1)
-      allocate(xyz(3*npt))
-      call make_grid(mb, xyz, npt)
-      call cnd_check(cnd_set_grid(ctx, int(mb, c_int), xyz), ctx, 'cnd_set_grid')
-      deallocate(xyz)
+      if (gridtype == 0) then          ! gridtoy, on the device
+         call cnd_check(cnd_make_grid(ctx, int(mb, c_int), 0_c_int, &
+                        [real(lxi0, nr), real(let0, nr), real(lze0, nr), doml0, doml1, domh, span], 7_c_int), &
+                        ctx, 'cnd_make_grid')
+      else if (gridtype == 1) then     ! gridtoy_multi, on the device
+         call cnd_check(cnd_make_grid(ctx, int(mb, c_int), 1_c_int, &
+                        [real(lxi0, nr), real(lxi1, nr), real(lxi2, nr), real(let0, nr), real(lze0, nr), &
+                         doml0, doml1, domh, span, real(mb, nr)], 10_c_int), ctx, 'cnd_make_grid')
+      else                             ! a grid file, uploaded
+         npt = (lximb(mb)+1) * (letmb(mb)+1) * (lzemb(mb)+1)
+         allocate(xyz(3*npt))
+         call make_grid(mb, xyz, npt)
+         call cnd_check(cnd_set_grid(ctx, int(mb, c_int), xyz), ctx, 'cnd_set_grid')
+         deallocate(xyz)
+      end if
    end do
    call cnd_check(cnd_init_metrics(ctx), ctx, 'cnd_init_metrics')
 
@@ -139,6 +155,14 @@ program canard_gpu
       deallocate(q)
    end do
    call cnd_check(cnd_set_clock(ctx, 0_c_int, int(n, c_int), int(ndt, c_int), dt, dts, dte, timo), ctx, 'cnd_set_clock')
+
+   ! ---- output: grid file, recording and the output times (mo_canard_driver.f90:173-195) --
+   if (ndata >= 0) then
+      call execute_command_line('mkdir -p data')
+      call cnd_check(cnd_output_grid(ctx, 0_c_int, 'data/grid.plt'//c_null_char), ctx, 'cnd_output_grid')
+      call cnd_check(cnd_record(ctx, 1_c_int), ctx, 'cnd_record')
+      call cnd_check(cnd_output_schedule(ctx, 0_c_int, tmax, int(ndata, c_int)), ctx, 'cnd_output_schedule')
+   end if
    if (ltimer) call cnd_check(cnd_profile(ctx, 1_c_int), ctx, 'cnd_profile')
 
    ! ---- time loop (mo_canard_driver.f90:226-431) ----------------------------------------
@@ -147,7 +171,18 @@ program canard_gpu
       write(*,"(' n =',i8,'   time =',f12.5)") n, timo
       call cnd_check(cnd_step(ctx, 1_c_int), ctx, 'cnd_step')
       call cnd_check(cnd_get_clock(ctx, 0_c_int, n, ndt, dt, dts, dte, timo), ctx, 'cnd_get_clock')
+      ! output files data/nNNN.plt, written by a host thread while the next
+      ! steps run (mo_canard_driver.f90:373-425)
+      if (ndata >= 0) then
+         call cnd_check(cnd_output_due(ctx, 0_c_int, due, ndati, tout), ctx, 'cnd_output_due')
+         if (due == 1) then
+            write(fname, "('data/n',i3.3,'.plt')") ndati
+            write(*,"(' output ',a,' at time ',f12.5)") trim(fname), tout
+            call cnd_check(cnd_output_write(ctx, 0_c_int, trim(fname)//c_null_char, 1_c_int), ctx, 'cnd_output_write')
+         end if
+      end if
    end do
+   if (ndata >= 0) call cnd_check(cnd_output_wait(ctx), ctx, 'cnd_output_wait')
 
    ! ---- restart files (mo_canard_driver.f90:436-438) ---------------------------------------
    if (nrestart == 1) then

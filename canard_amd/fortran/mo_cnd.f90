@@ -115,6 +115,41 @@ module mo_cnd
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function
+      integer(c_int) function cnd_make_grid(ctx, b, kind, par, npar) bind(C, name="cnd_make_grid")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: b, kind, npar
+         real(c_double), intent(in) :: par(*)
+      end function
+      integer(c_int) function cnd_output_schedule(ctx, inst, tmax, ndata) bind(C, name="cnd_output_schedule")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: inst, ndata
+         real(c_double), value :: tmax
+      end function
+      integer(c_int) function cnd_output_due(ctx, inst, due, ndati, time) bind(C, name="cnd_output_due")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: inst
+         integer(c_int), intent(out) :: due, ndati
+         real(c_double), intent(out) :: time
+      end function
+      integer(c_int) function cnd_output_write(ctx, inst, path, async) bind(C, name="cnd_output_write")
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: inst, async
+         character(kind=c_char), intent(in) :: path(*)
+      end function
+      integer(c_int) function cnd_output_grid(ctx, inst, path) bind(C, name="cnd_output_grid")
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: inst
+         character(kind=c_char), intent(in) :: path(*)
+      end function
+      integer(c_int) function cnd_output_wait(ctx) bind(C, name="cnd_output_wait")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
       integer(c_int) function cnd_use_graph(ctx, enable) bind(C, name="cnd_use_graph")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx

@@ -99,6 +99,22 @@ class Case:
             self._check(self.L.cnd_set_grid(self.h, b, _dp(a)), "cnd_set_grid")
         self._check(self.L.cnd_init_metrics(self.h), "cnd_init_metrics")
 
+    def make_grid(self, b, kind, *par):
+        """Generate block b's grid on the device: kind 'toy' (lxi0, let0,
+        lze0, doml0, doml1, domh, span), 'toy_multi' (lxi0, lxi1, lxi2, let0,
+        lze0, doml0, doml1, domh, span, mb) or 'sheared' (x0, h, amp, lx, ly);
+        call init_metrics once every block has its grid."""
+        a = np.ascontiguousarray(par, dtype=np.float64)
+        self._check(self.L.cnd_make_grid(self.h, b, _lib.GRIDS[kind], _dp(a), len(a)), "cnd_make_grid")
+
+    def init_metrics(self):
+        self._check(self.L.cnd_init_metrics(self.h), "cnd_init_metrics")
+
+    def grid(self, b):
+        xyz = np.empty(3 * self.npt[b])
+        self._check(self.L.cnd_get_grid(self.h, b, _dp(xyz)), "cnd_get_grid")
+        return xyz.reshape(3, self.npt[b])
+
     def metrics(self, b):
         n = self.npt[b]
         xim, etm, zem, yaco = np.empty(3 * n), np.empty(3 * n), np.empty(3 * n), np.empty(n)
@@ -152,6 +168,28 @@ class Case:
 
     def record_reset(self):
         self._check(self.L.cnd_record_reset(self.h), "cnd_record_reset")
+
+    # -- output (Tecplot files) ------------------------------------------------
+    def output_schedule(self, tmax, ndata, inst=0):
+        """Output k is due on the step whose interval holds k * tmax / ndata."""
+        self._check(self.L.cnd_output_schedule(self.h, inst, float(tmax), int(ndata)), "cnd_output_schedule")
+
+    def output_due(self, inst=0):
+        """(due, ndati, time) of the step just taken."""
+        due, nd, t = C.c_int(), C.c_int(), C.c_double()
+        self._check(self.L.cnd_output_due(self.h, inst, C.byref(due), C.byref(nd), C.byref(t)), "cnd_output_due")
+        return bool(due.value), nd.value, t.value
+
+    def output_write(self, path, inst=0, wait=True):
+        """Tecplot file of the recorded output; wait=False writes it from a
+        host thread while the device goes on (join with output_wait)."""
+        self._check(self.L.cnd_output_write(self.h, inst, str(path).encode(), int(not wait)), "cnd_output_write")
+
+    def output_grid(self, path, inst=0):
+        self._check(self.L.cnd_output_grid(self.h, inst, str(path).encode()), "cnd_output_grid")
+
+    def output_wait(self):
+        self._check(self.L.cnd_output_wait(self.h), "cnd_output_wait")
 
     def use_graph(self, on=True):
         self._check(self.L.cnd_use_graph(self.h, int(bool(on))), "cnd_use_graph")
@@ -220,3 +258,18 @@ class Case:
         pf = (C.POINTER(C.c_double) * len(fs))(*[_dp(a) for a in fs])
         self._check(self.L.cnd_filter_sweep(self.h, int(direction), pf), "cnd_filter_sweep")
         return fs
+
+
+def write_plt(path, title, names, zones, time=0.0, file_type=2):
+    """Tecplot binary file (cnd_write_plt): zones is a list of (dims, data)
+    with dims = (imax, jmax, kmax) and data a (len(names), points) array of
+    FP32 values, i fastest."""
+    L = _lib.load()
+    arrs = [np.ascontiguousarray(d, dtype=np.float32) for _, d in zones]
+    dims = (C.c_int * (3 * len(zones)))(*[int(v) for dz, _ in zones for v in dz])
+    nm = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    dp = (C.POINTER(C.c_float) * len(arrs))(*[a.ctypes.data_as(C.POINTER(C.c_float)) for a in arrs])
+    rc = L.cnd_write_plt(str(path).encode(), title.encode(), int(file_type), len(names), nm, len(zones), dims,
+                         float(time), dp)
+    if rc != 0:
+        raise CanardError(f"cnd_write_plt: {L.cnd_last_error(None).decode()} ({rc})")

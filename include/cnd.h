@@ -85,6 +85,20 @@ int cnd_set_grid(cnd_ctx *ctx, int b, const double *xyz);
  * mo_gcbc.f90:40-131), sponge lists (spongeup, mo_sponge.f90:34-71) and
  * junction tables (search_point/search_line, mo_domdcomp.f90:269-437).     */
 int cnd_init_metrics(cnd_ctx *ctx);
+/* Grid of block b generated on the device instead of uploaded
+ * (mo_gridgen.f90:510-654, makegrid's gridtoy and gridtoy_multi), or the
+ * sinusoidally sheared Cartesian block of the multi-block benchmark:
+ *  CND_GRID_TOY:       par = lxi0, let0, lze0, doml0, doml1, domh, span;
+ *      x = -doml0 + i (doml1 + doml0)/lxi0, y = -domh + j 2 domh/let0,
+ *      z = span ((lze0 - k)/lze0 - 1/2)
+ *  CND_GRID_TOY_MULTI: par = lxi0, lxi1, lxi2, let0, lze0, doml0, doml1,
+ *      domh, span, mb (block mb = 0..5 of the 3 x 2 arrangement)
+ *  CND_GRID_SHEARED:   par = x0, h, amp, lx, ly;
+ *      x = x0 + i h + amp lx sin(2 pi j h / ly), y = j h, z = k h
+ * cnd_get_grid copies block b's coordinates back, xyz(0:lmx,3).           */
+enum { CND_GRID_TOY = 0, CND_GRID_TOY_MULTI = 1, CND_GRID_SHEARED = 2 };
+int cnd_make_grid(cnd_ctx *ctx, int b, int kind, const double *par, int npar);
+int cnd_get_grid(cnd_ctx *ctx, int b, double *xyz);
 int cnd_get_metrics(cnd_ctx *ctx, int b, double *xim, double *etm,
                     double *zem, double *yaco);
 
@@ -156,6 +170,42 @@ int cnd_profile_read(cnd_ctx *ctx, int section, double *ms, long long *launches)
 int cnd_record(cnd_ctx *ctx, int on);
 int cnd_record_read(cnd_ctx *ctx, int block, float *out);
 int cnd_record_reset(cnd_ctx *ctx);
+
+/* Output of results (mo_canard_driver.f90:173-195, 286-291, 373-425;
+ * write_output_file / techead, mo_io.f90:345-529).
+ * cnd_output_schedule sets instance inst's output times (nml_canard tmax,
+ * ndata): output k (k = 0, 1, ...) is due on the step whose interval
+ * [timo, timo + dt] holds k tmax / ndata, decided on the device at the
+ * step's first stage; ndata < 1 turns it off (the default).  The count
+ * restarts at -1, as the reference's ndati does at start-up.
+ * cnd_output_due tells whether the step just taken was such a step (*due
+ * 1 or 0), the output's number (the reference's ndati) and its time
+ * timo - dtsum/2; synchronizes.
+ * cnd_output_write writes the Tecplot file of instance inst's recorded
+ * output (one zone per block, variables rho, u, v, w, p in FP32 as
+ * cnd_record_read forms them, title nNNN) and then clears the instance's
+ * time sums, as the reference does after an output.  async = 0 writes
+ * before returning; async = 1 returns at once: the variables are formed on
+ * the device in stream order, copied to pinned host memory on a second
+ * stream and written by a host thread while later steps run (the I/O
+ * server ranks of mo_io_server.f90:224-261); cnd_output_wait joins the
+ * write and returns its status.  Any later output call waits for it too.
+ * cnd_output_grid writes the grid file (file type 1, title "grid",
+ * variables x, y, z) of instance inst.                                     */
+int cnd_output_schedule(cnd_ctx *ctx, int inst, double tmax, int ndata);
+int cnd_output_due(cnd_ctx *ctx, int inst, int *due, int *ndati, double *time);
+int cnd_output_write(cnd_ctx *ctx, int inst, const char *path, int async);
+int cnd_output_grid(cnd_ctx *ctx, int inst, const char *path);
+int cnd_output_wait(cnd_ctx *ctx);
+/* Tecplot binary (#!TDV112) file with nzones ordered zones, host data:
+ * file_type 1 (grid) or 2 (solution), nvar variable names, dims[3 z + d]
+ * points of zone z along d, data[z] = nvar x points FP32 values, variable
+ * after variable, i fastest; zones are named z000, z001, ... with strand
+ * z + 1 and the given solution time, and carry each variable's minimum and
+ * maximum (techead + vminmax, mo_io.f90:417-562).  No device work.         */
+int cnd_write_plt(const char *path, const char *title, int file_type, int nvar,
+                  const char *const *names, int nzones, const int *dims,
+                  double time, const float *const *data);
 
 /* Sweep-shape autotuner (no counterpart in the reference, which has no
  * device kernels).  Kinds: CND_KIND_FILTER (filter phase), CND_KIND_GRAD

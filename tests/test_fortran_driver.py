@@ -12,17 +12,17 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "canard_amd", "fortran", "canard_gpu")
 
 
-def write_input(path, n=16, maxsteps=5, nts=0, tgv=True, extra=""):
+def write_input(path, n=16, maxsteps=5, nts=0, tgv=True, extra="", ndata=-1, tmax=100.0):
     txt = f"""&nml_driver
 nio = 0
 mbk = 0
-ndata = -1
+ndata = {ndata}
 /
 &nml_canard
 nts = {nts}
 nrestart = 1
 cfl = 0.8
-tmax = 100.0
+tmax = {tmax}
 ltimer = .true.
 /
 &nml_domdcomp
@@ -136,3 +136,45 @@ def test_driver_matches_library_and_restarts_bitwise(tmp_path):
     q_r, clock_r = cases.read_restart(str(b / "misc" / "restartz000.dat"), npt)
     assert clock_r == clock_f
     assert np.array_equal(q_r, q_f)
+
+
+@pytest.mark.gpu
+def test_driver_writes_tecplot_output(tmp_path):
+    """With ndata >= 0 the driver writes data/grid.plt at set-up and
+    data/nNNN.plt whenever a step crosses k tmax / ndata, from the recorded
+    averages (asynchronously); the files hold what the Python binding's
+    recording of the same run gives."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from plt_reader import read_plt
+    from canard_amd.solver import Case
+    n = 16
+    write_input(str(tmp_path), n=n, maxsteps=12, ndata=40, tmax=4.0)
+    r = run(str(tmp_path))
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    xyz = cases.gridtoy(n, n, n)
+    g = read_plt(str(tmp_path / "data" / "grid.plt"))
+    assert np.array_equal(g["zones"][0]["data"], xyz.astype(np.float32))
+    files = sorted(f for f in os.listdir(tmp_path / "data") if f.startswith("n"))
+    assert len(files) >= 2 and files[0] == "n000.plt"
+    c = Case([cases.periodic_block(n)], dict(reoo=16000.0))
+    c.set_grids([xyz])
+    c.set_states([cases.tgv_state(xyz, (n + 1,) * 3, mach=0.1)])
+    c.set_clock()
+    c.record(True)
+    c.output_schedule(4.0, 40)
+    got = {}
+    for _ in range(12):
+        c.step(1)
+        due, nd, t = c.output_due()
+        if due:
+            got[f"n{nd:03d}.plt"] = (t, c.record_read(0))
+            c.output_write(tmp_path / "py.plt")
+    c.close()
+    assert sorted(got) == files
+    for f in files:
+        z = read_plt(str(tmp_path / "data" / f))["zones"][0]
+        t, rec = got[f]
+        assert abs(z["time"] - t) < 1e-13
+        assert np.max(np.abs(z["data"] - rec)) <= 1e-6 * np.max(np.abs(rec))

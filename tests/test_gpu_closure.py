@@ -104,3 +104,36 @@ def test_box_filter_and_apply():
     assert np.max(np.abs(d - dfull)) <= 1e-6 * np.max(np.abs(dfull)) + 1e-15
     for c in (fine, coarse, net):
         c.close()
+
+
+def test_apply_to_two_contexts_in_a_row_equals_serial_applies():
+    """One network applied to two contexts back to back (each apply runs on
+    its context's stream, sharing the network's activation buffers) gives
+    bit for bit what two applies separated by a synchronisation give."""
+    from canard_amd.solver import Case
+    n = 32
+    g = cases.gridtoy(n, n, n)
+    states = [cases.tgv_state(g, (n + 1,) * 3, mach=0.2, pert=1e-2, seed=s) for s in (1, 2)]
+    net = _net(n, seed=5)
+    net.set_norm(np.array([1.0, 0, 0, 0, 1.8]), np.array([0.1, 0.2, 0.2, 0.2, 0.3]), np.full(5, 1e-2))
+
+    def make(q):
+        c = Case([cases.periodic_block(n)], dict(reoo=1000.0))
+        c.set_grids([g])
+        c.set_states([q])
+        return c
+
+    a = [make(q) for q in states]
+    for c in a:
+        net.apply(c, 0)           # no synchronisation between the two
+    got = [c.state(0) for c in a]
+    b = [make(q) for q in states]
+    want = []
+    for c in b:
+        net.apply(c, 0)
+        want.append(c.state(0))   # synchronises
+    for x, y in zip(got, want):
+        assert np.array_equal(x, y)
+    assert not np.array_equal(got[0], states[0])
+    for c in a + b + [net]:
+        c.close()

@@ -61,7 +61,9 @@ def test_create_rejects_unknown_precision():
 
 
 def test_calls_out_of_order_and_bad_arguments():
-    """Steps before the metrics, stages out of order, and bad indices are
+    """Steps before the metrics, stages out of order (the filter twice, a
+    stage other than the next one, the end of a step before its four stages,
+    a stage after a finished step without the filter), and bad indices are
     refused without touching the state; the context stays usable."""
     from canard_amd.solver import Case, CanardError
     blocks, grids, states, params = tgv_case(16, pert=1e-3)
@@ -81,14 +83,33 @@ def test_calls_out_of_order_and_bad_arguments():
         c.set_clock(inst=3)
     with pytest.raises(CanardError, match="bad argument"):
         c.set_shape("grad", 3, 0)
+    with pytest.raises(CanardError, match="four stages of the step have not run"):
+        c.end_step()
+    with pytest.raises(CanardError, match="stage 2 out of order"):
+        c.rk_stage(2)
     c.filter()
     with pytest.raises(CanardError, match="already ran"):
         c.filter()
     with pytest.raises(CanardError, match="half done"):
         c.step(1)
-    for nk in range(1, 5):
+    with pytest.raises(CanardError, match="four stages of the step have not run"):
+        c.end_step()
+    c.rk_stage(1)
+    with pytest.raises(CanardError, match="expected cnd_rk_stage\\(2\\)"):
+        c.end_step()
+    with pytest.raises(CanardError, match="stage 3 out of order"):
+        c.rk_stage(3)
+    with pytest.raises(CanardError, match="half done"):
+        c.filter()
+    with pytest.raises(CanardError, match="half done"):
+        c.record(True)
+    for nk in range(2, 5):
         c.rk_stage(nk)
+    with pytest.raises(CanardError, match="stage 1 out of order|half done"):
+        c.rk_stage(1)
     c.end_step()
+    with pytest.raises(CanardError, match="stage 2 out of order"):
+        c.rk_stage(2)
     c.step(1)
     q = c.state(0)
     assert np.all(np.isfinite(q))
