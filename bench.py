@@ -12,7 +12,7 @@ state resident in HBM.
 Prints one JSON line.  value = nominal cells x 4 RK stages x K / seconds.
 With --compare-mixed 1 (the default for an FP64 run) the same workload is
 then run in mixed precision (FP32 line solves with FP64 refinement, FP32
-gradients, fluxes and right-hand sides) for the same W + K steps; the line's "mixed" object
+gradients) for the same W + K steps; the line's "mixed" object
 gives its rate, its speed-up over FP64 and the max-norm relative error of
 each conserved variable against the FP64 state after W + K steps.
 """
@@ -260,7 +260,7 @@ def main():
     mixed = None
     if prec == CND_FP64 and a.compare_mixed:
         print(f"[bench] {n}^3 TGV, mixed: set-up", file=sys.stderr, flush=True)
-        dtm, _, _, qm, _, _ = run(CND_MIXED)
+        dtm, msec, _, qm, _, _ = run(CND_MIXED)
         err = []
         for m in range(5):   # q is (5, points)
             ref = np.max(np.abs(q[1:4])) if 1 <= m <= 3 else np.max(np.abs(q[m]))
@@ -268,10 +268,11 @@ def main():
         mixed = {"value": cells * 4 * a.steps / dtm, "ms_per_step": dtm / a.steps * 1e3,
                  "speedup_vs_fp64": dt / dtm, "steps_compared": a.warmup + a.steps,
                  "max_rel_err_per_variable": err, "error_target": 1e-6,
-                 "note": "FP32 line solves with one FP64 residual refinement per sweep; gradients, stored fluxes "
-                         "and right-hand-side buffers in FP32; state, metrics, Jacobian, RK update and GCBC in "
-                         "FP64. Error: max-norm over the field, momentum components against the largest momentum "
-                         "component."}
+                 "sections_ms_per_step": {k: v[0] / a.steps for k, v in msec.items()},
+                 "note": "FP32 line solves with one FP64 residual refinement per sweep and FP32 velocity and "
+                         "temperature gradients; state, metrics, fluxes, right-hand sides, RK update and GCBC in "
+                         "FP64. Error: max-norm over the field after warmup + steps steps against the FP64 run, "
+                         "momentum components against the largest momentum component."}
         del qm
     value = cells * 4 * a.steps / dt
     # dominant kernel group by measured time

@@ -21,7 +21,7 @@ def _dp(a):
 
 
 class Net:
-    """Network for an n^3 periodic coarse grid (n a multiple of 4) with room
+    """Network for an n^3 periodic coarse grid (n a multiple of 8) with room
     for `capacity` training pairs on the device."""
 
     def __init__(self, n, capacity=0, seed=0):

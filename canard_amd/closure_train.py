@@ -105,6 +105,7 @@ def norm_stats(ins, outs):
 
 
 def main(argv=None):
+    global MODES
     ap = argparse.ArgumentParser()
     ap.add_argument("--fine", type=int, default=256)
     ap.add_argument("--coarse", type=int, default=64)
@@ -122,7 +123,6 @@ def main(argv=None):
                          "998, its own fine truth); the judge uses the one with the lowest kinetic-energy error")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
-    global MODES
     MODES = a.modes
     nf, n = a.fine, a.coarse
     r = nf // n

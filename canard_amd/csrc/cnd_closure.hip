@@ -4,14 +4,15 @@
 // runs.  It has no counterpart in the reference; the compact filter stays in
 // every step, so a context without a network is the plain scheme.
 //
-// Every convolution is an implicit GEMM on the BF16 matrix cores: one wave
-// computes a 32 (voxels) x 32 (channels) tile with
-// v_mfma_f32_32x32x8bf16_1k (FP32 accumulate), stepping K over the 27 taps x
-// input channels; activations are BF16 channel-last [voxel][channel].
+// Every convolution is an implicit GEMM on the gfx950 BF16 matrix cores: one
+// wave computes a 32 (voxels) x 32 (channels) tile with
+// v_mfma_f32_32x32x16_bf16 (FP32 accumulate), stepping K over the 27 taps x
+// input channels, its workgroup's input halo and weights staged in LDS;
+// activations are BF16 channel-last [voxel][channel].
 //   forward        out[v][o]  = sum_{t,c} a[nb(v,t)][c] W[o][t][c] (+ b[o])
 //   data gradient  g[v][c]    = sum_{t,o} go[nb(v,t)][o] W[o][26-t][c]
-//   weight grad.   gW[o][t][c] = sum_v go[v][o] a[nb(v,t)][c]   (split over
-//                  voxel chunks, FP32 atomics)
+//   weight grad.   gW[o][t][c] = sum_v go[v][o] a[nb(v,t)][c]   (partial sums
+//                  per voxel chunk, then summed over the chunks in order)
 // Parameters are FP32 (Adam); BF16 copies in both layouts are rebuilt after
 // every update.  The grid is periodic, n^3 voxels (the context's stored
 // duplicate planes are filled from their originals).
@@ -33,7 +34,6 @@ using namespace cnd;
 namespace {
 
 using bf16 = unsigned short;   // bfloat16 bit pattern
-typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
 constexpr int kC0 = 5, kC0P = 8, kC1 = 32, kC2 = 32, kC3 = 5, kCP = 32, kTaps = 27;
@@ -42,7 +42,7 @@ constexpr int kC0 = 5, kC0P = 8, kC1 = 32, kC2 = 32, kC3 = 5, kCP = 32, kTaps = 
 constexpr int kOW1 = 0, kOB1 = kOW1 + kC1 * kTaps * kC0, kOW2 = kOB1 + kC1, kOB2 = kOW2 + kC2 * kTaps * kC1,
               kOW3 = kOB2 + kC2, kOB3 = kOW3 + kC3 * kTaps * kC2, kNParam = kOB3 + kC3;
 constexpr int kM1 = kTaps * kC0P, kM2 = kTaps * kCP;   // weight-gradient row lengths
-constexpr int kChunk = 2048;                           // voxels per wave in the weight gradient
+constexpr int kChunk = 2048;                           // voxels per wave (chunk) in the weight gradient
 
 __host__ __device__ inline uint32_t f2u(float f)
 {
@@ -74,39 +74,108 @@ __device__ inline int nbr(int x, int y, int z, int t, int n)
     return wrap(x + dx, n) + n * (wrap(y + dy, n) + n * wrap(z + dz, n));
 }
 
-// One wave per 32 voxels x 32 output channels.  EPI 0: + bias, ReLU, BF16
-// [V][32]; EPI 1: (+ bias) FP32 [V][ostride] for o < cout; EPI 2: times
-// (mask[v][o] > 0), BF16 [V][32].
-template <int CIP, int EPI>
-__global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const bf16 *__restrict__ w,
-                                              const float *__restrict__ bias, const bf16 *__restrict__ mask,
-                                              void *__restrict__ out, int n, int cout, int ostride)
+// Convolution as an implicit GEMM on v_mfma_f32_32x32x16_bf16 (gfx950: 16
+// K values per instruction, 8 per lane).  A workgroup of 8 waves owns a
+// tile of X (x) by 2R (y) by 4 (z) voxels, X = 32, 16 or 8 dividing n and
+// R = 32 / X; wave (wy, wz) computes the 32 voxels of its R rows of X at
+// plane wz against all 32 output channels.  The tile's input halo (one voxel
+// each side, periodic) and the layer's weights are staged in LDS first, so
+// the 27 taps read LDS only; voxel and weight rows are padded by 8 BF16
+// values, which spreads a wave's 16-byte reads over distinct banks.
+//   K order: for CIP = 32, tap t and channel block cb of 16 (lanes kh = 0, 1
+//   take its halves); for CIP = 8, one instruction covers the 8 channels of
+//   taps 2 tt (lanes kh = 0) and 2 tt + 1 (kh = 1), the 28th tap being zero.
+// EPI 0: + bias, ReLU, BF16 [V][32]; EPI 1: (+ bias) FP32 [V][ostride] for
+// o < cout; EPI 2: times (mask[v][o] > 0), BF16 [V][32].
+constexpr int kConvWaves = 8;
+
+__host__ __device__ constexpr int conv_vs(int cip) { return cip + 8; }           // LDS BF16 per halo voxel
+__host__ __device__ constexpr int conv_ws(int cip) { return kTaps * cip + 8; }   // LDS BF16 per weight row
+// x extent of a wave's voxel tile for an n^3 grid (n a multiple of 8)
+__host__ __device__ inline int conv_x(int n) { return n % 32 == 0 ? 32 : (n % 16 == 0 ? 16 : 8); }
+__host__ inline size_t conv_lds_bytes(int cip, int n)
 {
-    const int V = n * n * n;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int v0 = (blockIdx.x * 4 + wave) * 32;
-    if (v0 >= V) return;   // V is a multiple of 32: whole waves leave
+    const int X = conv_x(n), R = 32 / X;
+    return sizeof(bf16) * (size_t(32) * conv_ws(cip) + size_t(X + 2) * (2 * R + 2) * 6 * conv_vs(cip));
+}
+
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f16v mfma16(const s8v &a, const s8v &b, const f16v &c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, b), c, 0, 0,
+                                                  0);
+}
+
+template <int CIP, int EPI>
+__global__ __launch_bounds__(64 * kConvWaves) void k_conv(const bf16 *__restrict__ in, const bf16 *__restrict__ w,
+                                                         const float *__restrict__ bias,
+                                                         const bf16 *__restrict__ mask, void *__restrict__ out, int n,
+                                                         int cout, int ostride)
+{
+    constexpr int VS = conv_vs(CIP), WS = conv_ws(CIP), KT = kTaps * CIP;
+    extern __shared__ unsigned short conv_lds[];
+    bf16 *sw = conv_lds;             // weights [32][WS]
+    bf16 *sa = conv_lds + 32 * WS;   // halo [HZ][HY][HX][VS]
+    const int X = conv_x(n), R = 32 / X;
+    const int TY = 2 * R, HX = X + 2, HY = TY + 2, HZ = 6;
+    int wg = blockIdx.x;
+    const int ntx = n / X, nty = n / TY;
+    const int x0 = (wg % ntx) * X;
+    wg /= ntx;
+    const int y0 = (wg % nty) * TY, z0 = (wg / nty) * 4;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int e = tid; e < 32 * (KT / 8); e += nthr) {
+        const int o = e / (KT / 8), k8 = e - o * (KT / 8);
+        *reinterpret_cast<s8v *>(sw + o * WS + 8 * k8) = *reinterpret_cast<const s8v *>(w + o * KT + 8 * k8);
+    }
+    const int HV = HX * HY * HZ;
+    for (int e = tid; e < HV * (CIP / 8); e += nthr) {
+        const int hv = e / (CIP / 8), c8 = e - hv * (CIP / 8);
+        const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+        const int64_t gv = wrap(x0 + hx - 1, n) + int64_t(n) * (wrap(y0 + hy - 1, n) + int64_t(n) * wrap(z0 + hz - 1, n));
+        *reinterpret_cast<s8v *>(sa + hv * VS + 8 * c8) = *reinterpret_cast<const s8v *>(in + gv * CIP + 8 * c8);
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wy = wave & 1, wz = wave >> 1;
     const int i = lane & 31, kh = lane >> 5;
-    const int va = v0 + i;
-    const int x = va % n, y = (va / n) % n, z = va / (n * n);
+    const int lx = i % X, ly = wy * R + i / X;   // voxel i of the wave in the tile (plane wz)
     f16v acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    const bf16 *wrow = w + i * (kTaps * CIP) + kh * 4;   // B column = output channel i
-    for (int t = 0; t < kTaps; ++t) {
-        const bf16 *arow = in + int64_t(nbr(x, y, z, t, n)) * CIP + kh * 4;
-        const bf16 *brow = wrow + t * CIP;
+    const bf16 *wrow = sw + i * WS;   // B column: output channel i
+    auto halo = [&](int t) { return ((wz + t / 9) * HY + (ly + (t / 3) % 3)) * HX + (lx + t % 3); };
+    if constexpr (CIP == 8) {
+#pragma unroll 2
+        for (int tt = 0; tt < (kTaps + 1) / 2; ++tt) {
+            const int t = 2 * tt + kh;
+            s8v a, b;
 #pragma unroll
-        for (int cb = 0; cb < CIP / 8; ++cb) {
-            const s4 a = *reinterpret_cast<const s4 *>(arow + cb * 8);
-            const s4 b = *reinterpret_cast<const s4 *>(brow + cb * 8);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+            for (int e = 0; e < 8; ++e) { a[e] = 0; b[e] = 0; }
+            if (t < kTaps) {
+                a = *reinterpret_cast<const s8v *>(sa + halo(t) * VS);
+                b = *reinterpret_cast<const s8v *>(wrow + t * CIP);
+            }
+            acc = mfma16(a, b, acc);
+        }
+    } else {
+        for (int t = 0; t < kTaps; ++t) {
+            const bf16 *arow = sa + halo(t) * VS + 8 * kh;
+            const bf16 *brow = wrow + t * CIP + 8 * kh;
+#pragma unroll
+            for (int cb = 0; cb < CIP / 16; ++cb)
+                acc = mfma16(*reinterpret_cast<const s8v *>(arow + 16 * cb), *reinterpret_cast<const s8v *>(brow + 16 * cb),
+                             acc);
         }
     }
     const int o = i;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const int64_t v = v0 + 8 * (r >> 2) + 4 * kh + (r & 3);
+        const int vl = 8 * (r >> 2) + 4 * kh + (r & 3);   // row of the result: voxel vl of the wave
+        const int gx = x0 + vl % X, gy = y0 + wy * R + vl / X, gz = z0 + wz;
+        const int64_t v = gx + int64_t(n) * (gy + int64_t(n) * gz);
         float val = acc[r];
         if constexpr (EPI == 0) {
             val += bias[o];
@@ -123,16 +192,21 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const
     }
 }
 
-// gw[o][m] += sum_v go[v][o] a[nb(v, m / CIP)][m % CIP]; grid (ceil(27 CIP /
-// 32), ceil(V / (4 kChunk))), four waves per workgroup on separate chunks.
+// Partial weight gradients of one voxel chunk: part[chunk][o][m] = sum over
+// the chunk's voxels v of go[v][o] a[nb(v, m / CIP)][m % CIP], on
+// v_mfma_f32_32x32x16_bf16 (K = 16 voxels per instruction); grid (ceil(27
+// CIP / 32), ceil(V / (4 kChunk))), four waves per workgroup on separate
+// chunks.  k_grad_reduce sums the chunks (a fixed order: no atomics, the
+// gradient is the same on every run).
 template <int CIP>
 __global__ __launch_bounds__(256) void k_wgrad(const bf16 *__restrict__ go, const bf16 *__restrict__ act,
-                                               float *__restrict__ gw, int n)
+                                               float *__restrict__ part, int n)
 {
     const int V = n * n * n, M = kTaps * CIP;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 31, kh = lane >> 5;
-    const int vb0 = (blockIdx.y * 4 + wave) * kChunk;
+    const int chunk = blockIdx.y * 4 + wave;
+    const int vb0 = chunk * kChunk;
     if (vb0 >= V) return;
     const int vend = min(V, vb0 + kChunk);
     const int m = blockIdx.x * 32 + i;
@@ -141,27 +215,28 @@ __global__ __launch_bounds__(256) void k_wgrad(const bf16 *__restrict__ go, cons
     f16v acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    for (int vb = vb0; vb < vend; vb += 8) {   // V and kChunk are multiples of 8
-        s4 a, b;
+    for (int vb = vb0; vb < vend; vb += 16) {   // V and kChunk are multiples of 16 (n a multiple of 8)
+        s8v a, b;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int v = vb + 4 * kh + e;
+        for (int e = 0; e < 8; ++e) {
+            const int v = vb + 8 * kh + e;
             a[e] = short(go[int64_t(v) * kCP + i]);
             const int x = v % n, y = (v / n) % n, z = v / (n * n);
             b[e] = mok ? short(act[int64_t(nbr(x, y, z, t, n)) * CIP + c]) : short(0);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        acc = mfma16(a, b, acc);
     }
     if (!mok) return;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int o = 8 * (r >> 2) + 4 * kh + (r & 3);
-        atomicAdd(&gw[o * M + m], acc[r]);
+        part[(int64_t(chunk) * kCP + o) * M + m] = acc[r];
     }
 }
 
-// gb[o] += sum_v go[v][o]; one thread per (voxel chunk, channel)
-__global__ __launch_bounds__(256) void k_bgrad(const bf16 *__restrict__ go, float *__restrict__ gb, int V)
+// Partial bias gradients: part[chunk][o] = sum over 256 voxels of go[v][o];
+// one thread per (chunk, channel).
+__global__ __launch_bounds__(256) void k_bgrad(const bf16 *__restrict__ go, float *__restrict__ part, int V)
 {
     const int o = threadIdx.x & 31;
     const int ch = blockIdx.x * 8 + (threadIdx.x >> 5);
@@ -170,7 +245,19 @@ __global__ __launch_bounds__(256) void k_bgrad(const bf16 *__restrict__ go, floa
     const int v1 = min(V, v0 + 256);
     float s = 0.f;
     for (int v = v0; v < v1; ++v) s += from_bf16(go[int64_t(v) * kCP + o]);
-    atomicAdd(&gb[o], s);
+    part[int64_t(ch) * kCP + o] = s;
+}
+
+// g[k] = sum over nch chunks of part[chunk][k], k < len (second level of the
+// gradient reductions).
+__global__ __launch_bounds__(256) void k_grad_reduce(const float *__restrict__ part, float *__restrict__ g, int len,
+                                                     int nch)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= len) return;
+    float s = 0.f;
+    for (int ch = 0; ch < nch; ++ch) s += part[int64_t(ch) * len + k];
+    g[k] = s;
 }
 
 // Mean-square loss over V x 5 values: g = 2 (y - t) / (5 V) as BF16 [V][32]
@@ -374,6 +461,7 @@ struct cnd_net {
     Norm nm{};
     float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;   // parameters, gradient, Adam moments
     float *gacc = nullptr;                                            // gw1 | gw2 | gw3 | gb (3 x 32)
+    float *gpart = nullptr, *bpart = nullptr;   // per-chunk partial weight and bias gradients
     Wts W{};
     bf16 *x0 = nullptr, *a1 = nullptr, *a2 = nullptr, *g3 = nullptr, *g2 = nullptr, *g1 = nullptr;
     float *y = nullptr, *gx = nullptr, *hx = nullptr;
@@ -408,6 +496,9 @@ float *gw3(cnd_net *t) { return t->gacc + kCP * kM1 + kCP * kM2; }
 float *gbias(cnd_net *t) { return t->gacc + kCP * kM1 + 2 * kCP * kM2; }
 constexpr int64_t kGacc = int64_t(kCP) * kM1 + 2 * int64_t(kCP) * kM2 + 3 * kCP;
 
+int net_begin(cnd_net *t, hipStream_t st);
+int net_end(cnd_net *t, hipStream_t st);
+
 int prep(cnd_net *t, hipStream_t st)
 {
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(kCP * kTaps * kCP, 256)), dim3(256), 0, st, t->p, t->W);
@@ -415,15 +506,35 @@ int prep(cnd_net *t, hipStream_t st)
     return CND_OK;
 }
 
-// forward of the input in x0 into y (normalised correction, FP32 [V][5])
-int forward(cnd_net *t, const bf16 *x0, hipStream_t st)
+// forward of the input in x0 into y (normalised correction, FP32 [V][5]),
+// with the hidden activations in a1, a2 (BF16 [V][32])
+int forward_into(cnd_net *t, const bf16 *x0, bf16 *a1, bf16 *a2, float *y, hipStream_t st)
 {
-    const dim3 gv(blocks_for(t->V, 128)), b(256);
-    hipLaunchKernelGGL((k_conv<kC0P, 0>), gv, b, 0, st, x0, t->W.wf1, t->W.bb1, nullptr, t->a1, t->n, kCP, kCP);
-    hipLaunchKernelGGL((k_conv<kCP, 0>), gv, b, 0, st, t->a1, t->W.wf2, t->W.bb2, nullptr, t->a2, t->n, kCP, kCP);
-    hipLaunchKernelGGL((k_conv<kCP, 1>), gv, b, 0, st, t->a2, t->W.wf3, t->W.bb3, nullptr, t->y, t->n, kC3, kC3);
+    const dim3 gv(unsigned(t->V / (32 * kConvWaves))), b(64 * kConvWaves);
+    const size_t l8 = conv_lds_bytes(kC0P, t->n), l32 = conv_lds_bytes(kCP, t->n);
+    hipLaunchKernelGGL((k_conv<kC0P, 0>), gv, b, l8, st, x0, t->W.wf1, t->W.bb1, nullptr, a1, t->n, kCP, kCP);
+    hipLaunchKernelGGL((k_conv<kCP, 0>), gv, b, l32, st, a1, t->W.wf2, t->W.bb2, nullptr, a2, t->n, kCP, kCP);
+    hipLaunchKernelGGL((k_conv<kCP, 1>), gv, b, l32, st, a2, t->W.wf3, t->W.bb3, nullptr, y, t->n, kC3, kC3);
     NHIP(t, hipGetLastError());
     return CND_OK;
+}
+
+// ... with the network's own buffers
+int forward(cnd_net *t, const bf16 *x0, hipStream_t st) { return forward_into(t, x0, t->a1, t->a2, t->y, st); }
+
+// q += correction(q) on the coarse block C with the buffers x0, a1, a2, y
+// (cnd_net_apply and its variant without allocations)
+int apply_into(cnd_net *t, const CtxView &C, bf16 *x0, bf16 *a1, bf16 *a2, float *y)
+{
+    int r;
+    if ((r = net_begin(t, C.st))) return r;
+    const int64_t V = t->V;
+    hipLaunchKernelGGL(k_pack_state, dim3(blocks_for(V, 256)), dim3(256), 0, C.st, C.q, C.NT, C.off, t->n, t->nm, x0);
+    if ((r = forward_into(t, x0, a1, a2, y, C.st))) return r;
+    const int64_t P = int64_t(t->n + 1) * (t->n + 1) * (t->n + 1);
+    hipLaunchKernelGGL(k_add_correction, dim3(blocks_for(P, 256)), dim3(256), 0, C.st, C.q, C.NT, C.off, t->n, t->nm,
+                       y);
+    return net_end(t, C.st);
 }
 
 // backward from g3 (dL/dy, BF16 [V][32]) with the activations of the last
@@ -431,23 +542,34 @@ int forward(cnd_net *t, const bf16 *x0, hipStream_t st)
 int backward(cnd_net *t, const bf16 *x0, float *gx, hipStream_t st)
 {
     const int V = t->V;
-    hipLaunchKernelGGL(k_fill, dim3(blocks_for(kGacc, 256)), dim3(256), 0, st, t->gacc, 0.f, kGacc);
-    const dim3 b(256), gv(blocks_for(V, 128));
-    const unsigned nch = blocks_for(V, 4 * kChunk);
-    const unsigned nbg = blocks_for(blocks_for(V, 256), 8);
+    const dim3 b(256), gv(unsigned(V / (32 * kConvWaves))), bc(64 * kConvWaves);
+    const size_t l32 = conv_lds_bytes(kCP, t->n);
+    const unsigned nchw = blocks_for(V, 4 * kChunk);   // workgroups of four chunks
+    const int nch = (V + kChunk - 1) / kChunk, nbch = (V + 255) / 256;
+    const unsigned nbg = blocks_for(nbch, 8);
+    auto wgrad_sum = [&](float *gw, int M) {
+        hipLaunchKernelGGL(k_grad_reduce, dim3(blocks_for(int64_t(kCP) * M, 256)), b, 0, st, t->gpart, gw, kCP * M, nch);
+    };
+    auto bgrad = [&](const bf16 *go, float *gb) {
+        hipLaunchKernelGGL(k_bgrad, dim3(nbg), b, 0, st, go, t->bpart, V);
+        hipLaunchKernelGGL(k_grad_reduce, dim3(1), b, 0, st, t->bpart, gb, kCP, nbch);
+    };
     // layer 3
-    hipLaunchKernelGGL(k_wgrad<kCP>, dim3(kM2 / 32, nch), b, 0, st, t->g3, t->a2, gw3(t), t->n);
-    hipLaunchKernelGGL(k_bgrad, dim3(nbg), b, 0, st, t->g3, gbias(t) + 2 * kCP, V);
-    hipLaunchKernelGGL((k_conv<kCP, 2>), gv, b, 0, st, t->g3, t->W.wd3, nullptr, t->a2, t->g2, t->n, kCP, kCP);
+    hipLaunchKernelGGL(k_wgrad<kCP>, dim3(kM2 / 32, nchw), b, 0, st, t->g3, t->a2, t->gpart, t->n);
+    wgrad_sum(gw3(t), kM2);
+    bgrad(t->g3, gbias(t) + 2 * kCP);
+    hipLaunchKernelGGL((k_conv<kCP, 2>), gv, bc, l32, st, t->g3, t->W.wd3, nullptr, t->a2, t->g2, t->n, kCP, kCP);
     // layer 2
-    hipLaunchKernelGGL(k_wgrad<kCP>, dim3(kM2 / 32, nch), b, 0, st, t->g2, t->a1, gw2(t), t->n);
-    hipLaunchKernelGGL(k_bgrad, dim3(nbg), b, 0, st, t->g2, gbias(t) + kCP, V);
-    hipLaunchKernelGGL((k_conv<kCP, 2>), gv, b, 0, st, t->g2, t->W.wd2, nullptr, t->a1, t->g1, t->n, kCP, kCP);
+    hipLaunchKernelGGL(k_wgrad<kCP>, dim3(kM2 / 32, nchw), b, 0, st, t->g2, t->a1, t->gpart, t->n);
+    wgrad_sum(gw2(t), kM2);
+    bgrad(t->g2, gbias(t) + kCP);
+    hipLaunchKernelGGL((k_conv<kCP, 2>), gv, bc, l32, st, t->g2, t->W.wd2, nullptr, t->a1, t->g1, t->n, kCP, kCP);
     // layer 1
-    hipLaunchKernelGGL(k_wgrad<kC0P>, dim3((kM1 + 31) / 32, nch), b, 0, st, t->g1, x0, gw1(t), t->n);
-    hipLaunchKernelGGL(k_bgrad, dim3(nbg), b, 0, st, t->g1, gbias(t), V);
+    hipLaunchKernelGGL(k_wgrad<kC0P>, dim3((kM1 + 31) / 32, nchw), b, 0, st, t->g1, x0, t->gpart, t->n);
+    wgrad_sum(gw1(t), kM1);
+    bgrad(t->g1, gbias(t));
     if (gx)
-        hipLaunchKernelGGL((k_conv<kCP, 1>), gv, b, 0, st, t->g1, t->W.wd1, nullptr, nullptr, gx, t->n, kC0, kC0);
+        hipLaunchKernelGGL((k_conv<kCP, 1>), gv, bc, l32, st, t->g1, t->W.wd1, nullptr, nullptr, gx, t->n, kC0, kC0);
     hipLaunchKernelGGL(k_gather_grad, dim3(blocks_for(kNParam, 256)), b, 0, st, gw1(t), gw2(t), gw3(t), gbias(t),
                        t->g);
     NHIP(t, hipGetLastError());
@@ -487,10 +609,20 @@ int cnd_net_create(int n, int capacity, unsigned long long seed, cnd_net **out)
 {
     if (!out) return nfail(nullptr, CND_EINVAL, "cnd_net_create: null output");
     *out = nullptr;
-    if (n < 4 || n % 4 != 0 || n > 512) return nfail(nullptr, CND_EINVAL, "cnd_net_create: n must be a multiple of 4 in 4..512");
+    if (n < 8 || n % 8 != 0 || n > 512) return nfail(nullptr, CND_EINVAL, "cnd_net_create: n must be a multiple of 8 in 8..512");
     if (capacity < 0) return nfail(nullptr, CND_EINVAL, "cnd_net_create: negative capacity");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return nfail(nullptr, CND_EDEVICE, "no HIP device");
+    {
+        int dev = 0, lds = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return nfail(nullptr, CND_EDEVICE, "cnd_net_create: device attributes");
+        if (conv_lds_bytes(kCP, n) > size_t(lds))
+            return nfail(nullptr, CND_EDEVICE, "cnd_net_create: the convolution tiles need " +
+                                                   std::to_string(conv_lds_bytes(kCP, n)) + " bytes of LDS, the device has " +
+                                                   std::to_string(lds));
+    }
     cnd_net *t = new cnd_net;
     t->n = n;
     t->V = n * n * n;
@@ -504,6 +636,8 @@ int cnd_net_create(int n, int capacity, unsigned long long seed, cnd_net **out)
     if (!good) r = nfail(t, CND_EDEVICE, "stream or event creation failed");
     good = good && ok(nalloc(t, &t->p, kNParam)) && ok(nalloc(t, &t->g, kNParam)) && ok(nalloc(t, &t->m1, kNParam)) &&
            ok(nalloc(t, &t->m2, kNParam)) && ok(nalloc(t, &t->gacc, kGacc)) && ok(nalloc(t, &t->W.wf1, S8)) &&
+           ok(nalloc(t, &t->gpart, int64_t((V + kChunk - 1) / kChunk) * kCP * kM2)) &&
+           ok(nalloc(t, &t->bpart, int64_t((V + 255) / 256) * kCP)) &&
            ok(nalloc(t, &t->W.wf2, S32)) && ok(nalloc(t, &t->W.wf3, S32)) && ok(nalloc(t, &t->W.wd1, S32)) &&
            ok(nalloc(t, &t->W.wd2, S32)) && ok(nalloc(t, &t->W.wd3, S32)) && ok(nalloc(t, &t->W.bb1, kCP)) &&
            ok(nalloc(t, &t->W.bb2, kCP)) && ok(nalloc(t, &t->W.bb3, kCP)) && ok(nalloc(t, &t->x0, V * kC0P)) &&
@@ -545,7 +679,7 @@ void cnd_net_destroy(cnd_net *t)
     if (!t) return;
     if (t->done) (void)hipEventSynchronize(t->done);
     if (t->st) (void)hipStreamSynchronize(t->st);
-    void *ptrs[] = {t->p, t->g, t->m1, t->m2, t->gacc, t->W.wf1, t->W.wf2, t->W.wf3, t->W.wd1, t->W.wd2, t->W.wd3,
+    void *ptrs[] = {t->p, t->g, t->m1, t->m2, t->gacc, t->gpart, t->bpart, t->W.wf1, t->W.wf2, t->W.wf3, t->W.wd1, t->W.wd2, t->W.wd3,
                     t->W.bb1, t->W.bb2, t->W.bb3, t->x0, t->a1, t->a2, t->g3, t->g2, t->g1, t->y, t->gx, t->hx,
                     t->loss, t->din, t->dtg};
     for (void *p : ptrs) if (p) (void)hipFree(p);
@@ -709,14 +843,7 @@ int cnd_net_apply(cnd_net *t, cnd_ctx *ctx, int b)
     CtxView C;
     int r;
     if ((r = coarse_view(t, ctx, b, C, "cnd_net_apply"))) return r;
-    if ((r = net_begin(t, C.st))) return r;
-    const int64_t V = t->V;
-    hipLaunchKernelGGL(k_pack_state, dim3(blocks_for(V, 256)), dim3(256), 0, C.st, C.q, C.NT, C.off, t->n, t->nm, t->x0);
-    if ((r = forward(t, t->x0, C.st))) return r;
-    const int64_t P = int64_t(t->n + 1) * (t->n + 1) * (t->n + 1);
-    hipLaunchKernelGGL(k_add_correction, dim3(blocks_for(P, 256)), dim3(256), 0, C.st, C.q, C.NT, C.off, t->n, t->nm,
-                       t->y);
-    return net_end(t, C.st);
+    return apply_into(t, C, t->x0, t->a1, t->a2, t->y);
 }
 
 int cnd_net_forward_host(cnd_net *t, const float *x, float *y)

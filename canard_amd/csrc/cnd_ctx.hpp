@@ -81,9 +81,6 @@ struct cnd_ctx {
     // mixed precision: FP32 gradients and stresses; FP32 LU / homogeneous
     // tables and the FP64 banded LHS for the refinement
     float *gradf = nullptr, *dlu32 = nullptr, *dseg32[kNumL] = {};
-    // mixed precision: the 15 stored fluxes (less the instance's offsets)
-    // and the three right-hand-side buffers in FP32
-    float *fl32 = nullptr, *de32 = nullptr;
     double *dlhs = nullptr;
     JuncDev jc, je;
     std::vector<char> grid_set;

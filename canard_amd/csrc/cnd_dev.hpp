@@ -51,14 +51,6 @@ struct InstDev {
     double umf[3], dudtmf[3];
     unsigned long long red[2];   // running maxima (bit patterns of >= 0 doubles)
     double dtsum;                // recorded time since the last reset (cnd_record)
-    // mixed precision: the fluxes of direction d are stored in FP32 less
-    // foff[d] (the inviscid flux at point g0, the instance's first point,
-    // set at the start of every stage), which takes the large free-stream
-    // part (the pressure term) out of the rounded values; a constant has a
-    // zero derivative, so the flux sweeps differentiate the stored values
-    // as they are
-    double foff[3][5];
-    int64_t g0;
     // output schedule (nml_canard ndata, tmax): output ndati is due on the
     // step whose interval holds ndati * tmax / ndata; nout = 1 on that step
     double tmax;

@@ -40,14 +40,13 @@ namespace {
 
 // Viscous stresses and heat flux (src/mo_physics.f90:245-299) and, from
 // them, the fluxes of all three directions (src/mo_physics.f90:319-382),
-// stored as F[5 d + m] for the flux sweeps.  TS: storage type of the
-// gradients and fluxes (float in mixed mode: the fluxes are then stored less
-// the instance's offsets foff); stresses and fluxes are computed in FP64.
-template <class TS>
+// stored as F[5 d + m] for the flux sweeps.  TG: storage type of the
+// gradients (float in mixed mode); stresses and fluxes are FP64.
+template <class TG>
 __global__ __launch_bounds__(kPts) void k_visc_flux(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
                                                    const Tile *__restrict__ tiles, const double *__restrict__ q,
-                                                   const TS *__restrict__ grad, const double *__restrict__ met,
-                                                   TS *__restrict__ F, int64_t NT)
+                                                   const TG *__restrict__ grad, const double *__restrict__ met,
+                                                   double *__restrict__ F, int64_t NT)
 {
     POINT_PROLOGUE
     const InstDev &I = inst[B.inst];
@@ -83,10 +82,7 @@ __global__ __launch_bounds__(kPts) void k_visc_flux(const BlkDev *__restrict__ b
         double o[5];
         dir_flux<true>(q1, q2, q3, q4, q5, u, v, w, p, m, I.umf, ts, o);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            if constexpr (std::is_same<TS, float>::value) F[(5 * d + c) * NT + g] = float(o[c] - I.foff[d][c]);
-            else F[(5 * d + c) * NT + g] = o[c];
-        }
+        for (int c = 0; c < 5; ++c) F[(5 * d + c) * NT + g] = o[c];
     }
 }
 
@@ -185,11 +181,8 @@ __device__ void movef(InstDev &I, double dtko, double dtk)
 }
 
 // Start of stage nk: frame velocity for fluxes (mo_canard_driver.f90:261-263)
-// and, in stage 1, clearing of the time-step maxima.  In mixed precision
-// also the flux offsets: the inviscid fluxes of the stage's state q at the
-// instance's first point.
-__global__ void k_stage_begin(InstDev *inst, int ninst, int nk, const double *q, const double *met, int64_t NT,
-                              int mixed)
+// and, in stage 1, clearing of the time-step maxima.
+__global__ void k_stage_begin(InstDev *inst, int ninst, int nk)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
@@ -197,17 +190,6 @@ __global__ void k_stage_begin(InstDev *inst, int ninst, int nk, const double *q,
     const int a = nk - 2 > 0 ? (nk - 2 < 1 ? nk - 2 : 1) : 0, c = nk - 1 < 1 ? nk - 1 : 1;
     movef(I, (I.dt * double(a)) / double(4 - nk + 3), (I.dt * double(c)) / double(4 - nk + 2));
     if (nk == 1) { I.red[0] = 0ull; I.red[1] = 0ull; }
-    if (mixed) {
-        const int64_t g = I.g0;
-        const double q1 = q[g], q2 = q[NT + g], q3 = q[2 * NT + g], q4 = q[3 * NT + g], q5 = q[4 * NT + g];
-        const double ri = 1.0 / q1;
-        const double u = q2 * ri, v = q3 * ri, w = q4 * ri;
-        const double p = kGm1 * (q5 - 0.5 * ((q2 * u + q3 * v) + q4 * w));
-        for (int d = 0; d < 3; ++d) {
-            const double m[3] = {met[(3 * d) * NT + g], met[(3 * d + 1) * NT + g], met[(3 * d + 2) * NT + g]};
-            dir_flux<false>(q1, q2, q3, q4, q5, u, v, w, p, m, I.umf, nullptr, I.foff[d]);
-        }
-    }
 }
 
 // dt schedule (mo_canard_driver.f90:278-285).
@@ -279,12 +261,10 @@ __global__ void k_record_clock(InstDev *inst, int ninst, int reset)
 }
 
 // Sponge source and RK update (mo_sponge.f90:75-88,
-// mo_canard_driver.f90:329-334): qa = qo - dt/(5-nk) * yaco * de.  TD:
-// storage type of the right-hand-side buffers (float in mixed mode).
-template <class TD>
+// mo_canard_driver.f90:329-334): qa = qo - dt/(5-nk) * yaco * de.
 __global__ __launch_bounds__(kPts) void k_update(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
                                                 const Tile *__restrict__ tiles, const double *cur,
-                                                const double *__restrict__ qo, const TD *__restrict__ de,
+                                                const double *__restrict__ qo, double *__restrict__ de,
                                                 const double *__restrict__ met, const double *__restrict__ sponge,
                                                 double *qa, int64_t NT, int nk)
 {
@@ -294,8 +274,7 @@ __global__ __launch_bounds__(kPts) void k_update(const BlkDev *__restrict__ blks
     const double r = dtk * met[9 * NT + g];
     double d[5];
 #pragma unroll
-    for (int m = 0; m < 5; ++m)
-        d[m] = (double(de[m * NT + g]) + double(de[(5 + m) * NT + g])) + double(de[(10 + m) * NT + g]);
+    for (int m = 0; m < 5; ++m) d[m] = (de[m * NT + g] + de[(5 + m) * NT + g]) + de[(10 + m) * NT + g];
     if (I.sponge) {
         const double a = sponge[g], bb = sponge[NT + g];
         if (a != 0.0) {
@@ -378,12 +357,10 @@ __device__ __forceinline__ double pressure(const double *q, int64_t NT, int64_t 
 
 // gcbc_setup: characteristic transform of the face derivative (BC 10) or of
 // the full right-hand side (walls, curvilinear interfaces).  grid (tiles, side).
-// TD: storage type of the right-hand-side buffers.
-template <class TD>
 __global__ __launch_bounds__(kLine) void k_gcbc_setup(const BlkDev *__restrict__ blks,
                                                      const InstDev *__restrict__ inst,
                                                      const Tile *__restrict__ tiles, int d, const double *q,
-                                                     const TD *de, const double *met, const double *cmf,
+                                                     const double *de, const double *met, const double *cmf,
                                                      double *drva, int64_t NT, int64_t FT)
 {
     const Tile t = tiles[blockIdx.x];
@@ -400,8 +377,7 @@ __global__ __launch_bounds__(kLine) void k_gcbc_setup(const BlkDev *__restrict__
     const Elem e = eleme(cm, q[g], q24, pressure(q, NT, g), inst[B.inst].umf);
     xtq2r(e, cm, xt);
     for (int c = 0; c < 5; ++c)
-        cha[c] = ra0 * drva[(side * 5 + c) * FT + fi] +
-                 ra1 * ((double(de[c * NT + g]) + double(de[(5 + c) * NT + g])) + double(de[(10 + c) * NT + g]));
+        cha[c] = ra0 * drva[(side * 5 + c) * FT + fi] + ra1 * ((de[c * NT + g] + de[(5 + c) * NT + g]) + de[(10 + c) * NT + g]);
     const double y = met[9 * NT + g];
     for (int c = 0; c < 5; ++c) yc[c] = y * cha[c];
     for (int r = 0; r < 5; ++r) {
@@ -412,12 +388,11 @@ __global__ __launch_bounds__(kLine) void k_gcbc_setup(const BlkDev *__restrict__
 }
 
 // gcbc_update of face (d, side); faces are applied in the reference's order.
-template <class TD>
 __global__ __launch_bounds__(kLine) void k_gcbc_update(const BlkDev *__restrict__ blks,
                                                       const InstDev *__restrict__ inst,
                                                       const TabDev *__restrict__ tabs,
                                                       const Tile *__restrict__ tiles, int d, int side,
-                                                      const double *q, TD *de, const double *met,
+                                                      const double *q, double *de, const double *met,
                                                       const double *cmf, const double *drva, int64_t NT, int64_t FT)
 {
     const Tile t = tiles[blockIdx.x];
@@ -485,10 +460,10 @@ __global__ __launch_bounds__(kLine) void k_gcbc_update(const BlkDev *__restrict_
         // the right-hand side is the sum of the three direction buffers: the
         // corrected sum goes to the first, the others are cleared
         for (int c = 0; c < 5; ++c) {
-            const double tot = (double(de[c * NT + gl]) + double(de[(5 + c) * NT + gl])) + double(de[(10 + c) * NT + gl]);
-            de[c * NT + gl] = TD(tot + sbcc * dha[c]);
-            de[(5 + c) * NT + gl] = TD(0);
-            de[(10 + c) * NT + gl] = TD(0);
+            const double tot = (de[c * NT + gl] + de[(5 + c) * NT + gl]) + de[(10 + c) * NT + gl];
+            de[c * NT + gl] = tot + sbcc * dha[c];
+            de[(5 + c) * NT + gl] = 0.0;
+            de[(10 + c) * NT + gl] = 0.0;
         }
     }
 }
@@ -698,9 +673,6 @@ int upload_inst(cnd_ctx *c)
         I.timf = p.timf; I.cfl = p.cfl; I.nsmf = p.nsmf; I.viscous = p.viscous; I.sponge = p.sponge;
         I.ndata = 0;
         I.ndati = -1;
-        I.g0 = -1;
-        for (int b = 0; b < c->nblk && I.g0 < 0; ++b)
-            if (c->spec[b].inst == i) I.g0 = c->off[b];
     }
     HIPX(c, hipMemcpy(c->dinst, h.data(), sizeof(InstDev) * c->ninst, hipMemcpyHostToDevice));
     return CND_OK;
@@ -785,9 +757,7 @@ int enqueue_stage(cnd_ctx *c, int nk)
 {
     const double *cur = nk == 1 ? c->qo : c->qa;
     const unsigned ib = unsigned((c->ninst + 63) / 64);
-    const bool mix = c->precision == CND_MIXED;
-    hipLaunchKernelGGL(k_stage_begin, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, nk, cur, c->met, c->NT,
-                       int(mix));
+    hipLaunchKernelGGL(k_stage_begin, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, nk);
     if (nk == 1) {
         Section sec(c, CND_SEC_TSTEP, 1);
         hipLaunchKernelGGL(k_tstep, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->met,
@@ -804,14 +774,14 @@ int enqueue_stage(cnd_ctx *c, int nk)
     }
     if (c->any_visc) {
         Section sec(c, CND_SEC_STRESS, 1);
-        if (mix)
+        if (c->precision == CND_MIXED)
             hipLaunchKernelGGL(k_visc_flux<float>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
-                               c->gradf, c->met, c->fl32, c->NT);
+                               c->gradf, c->met, c->fl, c->NT);
         else
             hipLaunchKernelGGL(k_visc_flux<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt,
                                cur, c->grad, c->met, c->fl, c->NT);
     }
-    const bool fused = fuse_possible(c) && (mix || seg_on(c, CND_KIND_FLUX, 2));
+    const bool fused = fuse_possible(c) && (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, 2));
     {
     Section sec(c, CND_SEC_FLUX, fused ? 2 : 3);
     for (int d = 0; d < (fused ? 2 : 3); ++d)
@@ -819,38 +789,22 @@ int enqueue_stage(cnd_ctx *c, int nk)
     }
     if (c->any_gcbc) {
         Section sec(c, CND_SEC_GCBC, 9);
-        for (int d = 0; d < 3; ++d) {
-            if (mix)
-                hipLaunchKernelGGL(k_gcbc_setup<float>, tiles_grid(c, d, 2), dim3(kLine), 0, c->st, c->dblk, c->dinst,
-                                   c->dtiles[d], d, cur, c->de32, c->met, c->cmf[d], c->drva[d], c->NT, c->FT[d]);
-            else
-                hipLaunchKernelGGL(k_gcbc_setup<double>, tiles_grid(c, d, 2), dim3(kLine), 0, c->st, c->dblk,
-                                   c->dinst, c->dtiles[d], d, cur, c->de, c->met, c->cmf[d], c->drva[d], c->NT,
-                                   c->FT[d]);
-        }
         for (int d = 0; d < 3; ++d)
-            for (int s = 0; s < 2; ++s) {
-                if (mix)
-                    hipLaunchKernelGGL(k_gcbc_update<float>, tiles_grid(c, d, 1), dim3(kLine), 0, c->st, c->dblk,
-                                       c->dinst, c->dtab, c->dtiles[d], d, s, cur, c->de32, c->met, c->cmf[d],
-                                       c->drva[d], c->NT, c->FT[d]);
-                else
-                    hipLaunchKernelGGL(k_gcbc_update<double>, tiles_grid(c, d, 1), dim3(kLine), 0, c->st, c->dblk,
-                                       c->dinst, c->dtab, c->dtiles[d], d, s, cur, c->de, c->met, c->cmf[d],
-                                       c->drva[d], c->NT, c->FT[d]);
-            }
+            hipLaunchKernelGGL(k_gcbc_setup, tiles_grid(c, d, 2), dim3(kLine), 0, c->st, c->dblk, c->dinst,
+                               c->dtiles[d], d, cur, c->de, c->met, c->cmf[d], c->drva[d], c->NT, c->FT[d]);
+        for (int d = 0; d < 3; ++d)
+            for (int s = 0; s < 2; ++s)
+                hipLaunchKernelGGL(k_gcbc_update, tiles_grid(c, d, 1), dim3(kLine), 0, c->st, c->dblk, c->dinst,
+                                   c->dtab, c->dtiles[d], d, s, cur, c->de, c->met, c->cmf[d], c->drva[d], c->NT,
+                                   c->FT[d]);
     }
     if (fused) {
         Section sec(c, CND_SEC_UPDATE, 1);
         if ((r = run_flux_update(c, nk, c->qa))) return r;
     } else {
         Section sec(c, CND_SEC_UPDATE, 1);
-        if (mix)
-            hipLaunchKernelGGL(k_update<float>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
-                               c->qo, c->de32, c->met, c->sponge, c->qa, c->NT, nk);
-        else
-            hipLaunchKernelGGL(k_update<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur,
-                               c->qo, c->de, c->met, c->sponge, c->qa, c->NT, nk);
+        hipLaunchKernelGGL(k_update, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->qo, c->de,
+                           c->met, c->sponge, c->qa, c->NT, nk);
     }
     hipLaunchKernelGGL(k_stage_mid, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, nk);
     Section avg(c, CND_SEC_AVG, 1);
@@ -1074,7 +1028,7 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     for (int li = 0; good && li < kNumL; ++li) good = ok(dalloc(c, &c->dseg[li], segtab[li].size()));
     if (good && c->any_sponge) good = ok(dalloc(c, &c->sponge, 2 * NT));
     if (good && precision == CND_MIXED)
-        good = ok(dalloc(c, &c->gradf, 12 * NT)) && ok(dalloc(c, &c->fl32, 15 * NT)) && ok(dalloc(c, &c->de32, 15 * NT)) &&
+        good = ok(dalloc(c, &c->gradf, 12 * NT)) &&
                ok(dalloc(c, &c->dlu32, lu.size())) && ok(dalloc(c, &c->dlhs, lhs.size()));
     for (int li = 0; good && precision == CND_MIXED && li < kNumL; ++li)
         good = ok(dalloc(c, &c->dseg32[li], segtab[li].size()));
@@ -1130,8 +1084,6 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     if (!r) {
         hipError_t e = hipMemset(c->de, 0, sizeof(double) * 15 * NT);
         if (e == hipSuccess) e = hipMemset(c->fl, 0, sizeof(double) * 15 * NT);
-        if (e == hipSuccess && c->de32) e = hipMemset(c->de32, 0, sizeof(float) * 15 * NT);
-        if (e == hipSuccess && c->fl32) e = hipMemset(c->fl32, 0, sizeof(float) * 15 * NT);
         if (e != hipSuccess) r = fail(c, CND_EDEVICE, "memset");
     }
     // junction plans (search_point / search_line)
@@ -1168,7 +1120,7 @@ void cnd_destroy(cnd_ctx *c)
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->fl, c->met, c->ys,
-                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->fl32, c->de32, c->dlu32, c->dlhs, c->jc.tgt, c->jc.src,
+                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->dlu32, c->dlhs, c->jc.tgt, c->jc.src,
                     c->jc.first, c->jc.scale, c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp,
                     c->qb};
     for (void *p : ptrs) if (p) hipFree(p);

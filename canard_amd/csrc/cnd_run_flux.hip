@@ -1,5 +1,4 @@
-// Flux derivative sweeps into the right-hand-side buffers (FP32 buffers of
-// FP32 stored fluxes in mixed precision).
+// Flux derivative sweeps into the right-hand-side buffers.
 #include "cnd_ctx.hpp"
 
 namespace cnd {
@@ -9,12 +8,10 @@ int run_flux(cnd_ctx *c, int d, const double *cur)
     const int K = CND_KIND_FLUX;
     const bool mix = c->precision == CND_MIXED;
     double *de = c->de + 5 * d * c->NT;
-    float *de32 = mix ? c->de32 + 5 * d * c->NT : nullptr;
     if (c->any_visc) {
         const double *F = c->fl + 5 * d * c->NT;
         if (mix)
-            LAUNCH_SEG(k_seg_flux_field, true, 1, c, K, d, 5, c->fl32 + 5 * d * c->NT, de32, c->drva[d], c->NT,
-                       c->FT[d]);
+            LAUNCH_SEG(k_seg_flux_field, true, 1, c, K, d, 5, F, de, c->drva[d], c->NT, c->FT[d]);
         else if (seg_on(c, K, d))
             LAUNCH_SEG(k_seg_flux_field, false, 1, c, K, d, 5, F, de, c->drva[d], c->NT, c->FT[d]);
         else
@@ -22,7 +19,7 @@ int run_flux(cnd_ctx *c, int d, const double *cur)
                          c->nmax[d], c->ys, F, de, c->drva[d], c->NT, c->FT[d]);
     } else {
         if (mix)
-            LAUNCH_SEG(k_seg_flux, true, 5, c, K, d, 1, c->dinst, cur, c->met, de32, c->drva[d], c->NT, c->FT[d]);
+            LAUNCH_SEG(k_seg_flux, true, 5, c, K, d, 1, c->dinst, cur, c->met, de, c->drva[d], c->NT, c->FT[d]);
         else if (seg_on(c, K, d))
             LAUNCH_SEG(k_seg_flux, false, 5, c, K, d, 1, c->dinst, cur, c->met, de, c->drva[d], c->NT, c->FT[d]);
         else

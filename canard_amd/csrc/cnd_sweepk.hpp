@@ -15,17 +15,17 @@
 namespace cnd {
 
 // ---- sources (values a sweep differentiates or filters) ---------------------
-template <int NV, class TI = double>
-struct SrcField {   // NV plain fields, vs apart, stored as TI
-    const TI *a;
+template <int NV>
+struct SrcField {   // NV plain fields, vs apart
+    const double *a;
     int64_t vs;
     struct Raw { double v[NV]; };
     static constexpr bool kDirect = true;   // variable v reads field v only
-    __device__ double field(int64_t g, int v) const { return double(a[v * vs + g]); }
+    __device__ double field(int64_t g, int v) const { return a[v * vs + g]; }
     __device__ void load(int64_t g, int, Raw &r) const
     {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) r.v[k] = double(a[k * vs + g]);
+        for (int k = 0; k < NV; ++k) r.v[k] = a[k * vs + g];
     }
     __device__ void finish(const Raw &r, int, double (&v)[NV]) const
     {
@@ -152,17 +152,15 @@ struct SinkFilt {  // out = f + correction
     __device__ void ends(int, int, int, double) {}
 };
 
-template <class TO = double>
-struct SinkFlux {  // the five components of dF/dxi_d (stored as TO); pencil ends to drva
-    TO *de;
-    double *drva;
+struct SinkFlux {  // the five components of dF/dxi_d; pencil ends to drva
+    double *de, *drva;
     int64_t NT, FT, fi0;
     __device__ void store(int64_t g, int, const double (&x)[5])
     {
 #pragma unroll
-        for (int m = 0; m < 5; ++m) de[m * NT + g] = TO(x[m]);
+        for (int m = 0; m < 5; ++m) de[m * NT + g] = x[m];
     }
-    __device__ void put(int64_t g, int, int m, double x) { de[m * NT + g] = TO(x); }
+    __device__ void put(int64_t g, int, int m, double x) { de[m * NT + g] = x; }
     static constexpr bool kReadBack = false;
     __device__ double src(int64_t, int) const { return 0.0; }
     __device__ double fold(double, double x) const { return x; }
@@ -170,14 +168,12 @@ struct SinkFlux {  // the five components of dF/dxi_d (stored as TO); pencil end
     __device__ void ends(int side, int lane, int m, double x) { drva[(side * 5 + m) * FT + fi0 + lane] = x; }
 };
 
-template <class TO = double>
-struct SinkFluxV {  // component v of dF/dxi_d (stored as TO); pencil ends to drva
-    TO *o;
-    double *drva;
+struct SinkFluxV {  // component v of dF/dxi_d; pencil ends to drva
+    double *o, *drva;
     int v;
     int64_t FT, fi0;
-    __device__ void store(int64_t g, int, const double (&x)[1]) { o[g] = TO(x[0]); }
-    __device__ void put(int64_t g, int, int, double x) { o[g] = TO(x); }
+    __device__ void store(int64_t g, int, const double (&x)[1]) { o[g] = x[0]; }
+    __device__ void put(int64_t g, int, int, double x) { o[g] = x; }
     static constexpr bool kReadBack = false;
     __device__ double src(int64_t, int) const { return 0.0; }
     __device__ double fold(double, double x) const { return x; }
@@ -190,19 +186,16 @@ struct SinkFluxV {  // component v of dF/dxi_d (stored as TO); pencil ends to dr
 // are bit-identical): qa = qo - dt/(5-nk) * yaco * ((de_xi + de_eta) + x).
 // Every operand is variable v at the sink's own point.  Pre holds one point's
 // operands, so sweeps can load them for several points before storing any.
-template <class TD = double>
 struct SinkUpdateV {
     double *qa, *drva;
-    const double *qo;
-    const TD *de0, *de1;   // the xi and eta right-hand-side buffers (stored as TD)
-    const double *met;
+    const double *qo, *de0, *de1, *met;
     double dtk;   // dt / (5 - nk) of the tile's instance
     int v;
     int64_t NT, FT, fi0;
     struct Pre { double s, q, r; };
     __device__ void pre(int64_t g, Pre &o) const
     {
-        o.s = double(de0[g]) + double(de1[g]);
+        o.s = de0[g] + de1[g];
         o.q = qo[g];
         o.r = dtk * met[9 * NT + g];
     }
@@ -283,7 +276,7 @@ __global__ __launch_bounds__(5 * kTile) void k_fd_flux(const BlkDev *__restrict_
     SWEEP_TILE_PROLOGUE(ntiles)
     const Tile t = tiles[tix];
     SrcFlux src{q, met, blks, inst, NT, d};
-    SinkFlux<> sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0};
+    SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0};
     sweep_tile<5, 0, C>(blks, tabs, lu, t, tix, d, 0, nmax, ys, src, sink, lds);
 }
 
@@ -299,7 +292,7 @@ __global__ __launch_bounds__(kTile) void k_fd_flux_field(const BlkDev *__restric
     const int v = blockIdx.y;
     const Tile t = tiles[tix];
     SrcField<1> src{F + v * NT, 0};
-    SinkFluxV<> sink{de + v * NT, drva, v, FT, blks[t.b].foff[d] + t.q0};
+    SinkFluxV sink{de + v * NT, drva, v, FT, blks[t.b].foff[d] + t.q0};
     sweep_tile<1, 0, C>(blks, tabs, lu, t, int64_t(v) * ntiles + tix, d, 0, nmax, ys, src, sink, lds);
 }
 
@@ -395,11 +388,6 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
     seg_tile<1, 1, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
 }
 
-// Storage type of the intermediate fields of a stage (gradients, stored
-// fluxes, right-hand-side buffers): FP32 in mixed mode.
-template <bool MIX>
-using TStore = typename std::conditional<MIX, float, double>::type;
-
 // Velocity and temperature gradients along d; in mixed mode they are stored
 // in FP32.
 template <class G, bool MIX>
@@ -418,26 +406,26 @@ __global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX), seg_waves<G>()) void
 template <class G, bool MIX>
 __global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *q, const double *met,
-    TStore<MIX> *de, double *drva, int64_t NT, int64_t FT)
+    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *q, const double *met, double *de,
+    double *drva, int64_t NT, int64_t FT)
 {
     SEG_PROLOGUE(ntiles)
     const Tile t = tiles[tix];
     SrcFlux src{q, met, blks, inst, NT, d};
-    SinkFlux<TStore<MIX>> sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0};
+    SinkFlux sink{de, drva, NT, FT, blks[t.b].foff[d] + t.q0};
     seg_tile<5, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
 }
 
 template <class G, bool MIX>
 __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, const TStore<MIX> *F, TStore<MIX> *de, double *drva, int64_t NT, int64_t FT)
+    int ntiles, int SP, int d, const double *F, double *de, double *drva, int64_t NT, int64_t FT)
 {
     SEG_PROLOGUE(ntiles)
     const int v = blockIdx.y;
     const Tile t = tiles[tix];
-    SrcField<1, TStore<MIX>> src{F + v * NT, 0};
-    SinkFluxV<TStore<MIX>> sink{de + v * NT, drva, v, FT, blks[t.b].foff[d] + t.q0};
+    SrcField<1> src{F + v * NT, 0};
+    SinkFluxV sink{de + v * NT, drva, v, FT, blks[t.b].foff[d] + t.q0};
     seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
 }
 
@@ -447,7 +435,7 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
 template <class G, bool MIX>
 __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_update(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
-    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const TStore<MIX> *F, const TStore<MIX> *de,
+    int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *F, const double *de,
     const double *__restrict__ qo, const double *__restrict__ met, double *qa, double *drva, int64_t NT, int64_t FT,
     int nk)
 {
@@ -458,9 +446,9 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
     extern __shared__ double lds[];
     const int tix = w / 5, v = w % 5;
     const Tile t = tiles[tix];
-    SrcField<1, TStore<MIX>> src{F + v * NT, 0};
+    SrcField<1> src{F + v * NT, 0};
     const double dtk = inst[blks[t.b].inst].dt / double(4 - nk + 1);
-    SinkUpdateV<TStore<MIX>> sink{qa + v * NT, drva, qo + v * NT, de + v * NT, de + (5 + v) * NT, met, dtk, v, NT, FT,
+    SinkUpdateV sink{qa + v * NT, drva, qo + v * NT, de + v * NT, de + (5 + v) * NT, met, dtk, v, NT, FT,
                      blks[t.b].foff[d] + t.q0};
     seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
 }

@@ -39,6 +39,10 @@ enum {
     CND_ENOMEM = -4
 };
 
+/* Precision of the step: CND_FP64; CND_MIXED, FP32 line solves with one
+ * FP64 residual refinement and the stage's velocity and temperature
+ * gradients stored in FP32 (state, metrics, fluxes, right-hand sides, RK
+ * update and characteristic boundaries stay FP64).                         */
 enum { CND_FP64 = 0, CND_MIXED = 1 };
 
 typedef struct cnd_ctx cnd_ctx;
@@ -244,7 +248,7 @@ int cnd_tuning(const cnd_ctx *ctx, int kind, int dir, int *shape, int *seg_len,
  * direction (periodic duplicates).                                         */
 typedef struct cnd_net cnd_net;
 int  cnd_net_param_count(void);
-/* n: coarse cells per direction (multiple of 4); capacity: training pairs
+/* n: coarse cells per direction (multiple of 8); capacity: training pairs
  * kept on the device; seed: initial weights.                               */
 int  cnd_net_create(int n, int capacity, unsigned long long seed, cnd_net **net);
 void cnd_net_destroy(cnd_net *net);

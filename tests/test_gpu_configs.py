@@ -55,10 +55,10 @@ def test_config3_multiblock_fp64_and_mixed():
     big = np.max([np.max(np.abs(qo - s0), axis=1) for qo, s0 in zip(qos, states)], axis=0)
     big[1:4] = np.max(big[1:4])
     for b in range(4):
-        qc, qo, qm = qcs[b], qos[b], m.state(b)
+        qc, qo = qcs[b], qos[b]
         e = rel_err(qc, qo)
         ec = np.max(np.abs((qc - states[b]) - (qo - states[b])), axis=1) / big
-        em = rel_err(qm, qc)
+        em = rel_err(m.state(b), qc)
         print(f"  block {b}: fp64 vs oracle {e}, change {ec}, mixed vs fp64 {em}", flush=True)
         assert np.all(e < 1e-12), (b, e)
         assert np.all(ec < 1e-9), (b, ec)
@@ -162,5 +162,5 @@ def test_config4_ensemble_fp64_graph_and_mixed():
         worst = np.maximum(worst, rel_err(m.state(i), c.state(i)))
     print("  mixed vs fp64, worst over instances", worst, flush=True)
     assert np.all(worst < 1e-6), worst
-    c.close()
-    m.close()
+    for x in (c, m):
+        x.close()

@@ -31,7 +31,7 @@ def test_gridtoy_and_metrics_match_the_uploaded_grid():
 
 def test_gridtoy_multi_six_blocks():
     from canard_amd.solver import Case
-    lxi0, lxi1, lxi2, let0, lze0 = 8, 12, 10, 9, 7
+    lxi0, lxi1, lxi2, let0, lze0 = 14, 18, 16, 13, 12
     blocks = []
     for mb in range(6):
         lxi = (lxi0, lxi1, lxi2)[mb % 3]
@@ -64,9 +64,9 @@ def test_sheared_pair_matches_host_generator():
 
 def test_bad_generator_arguments_are_refused():
     from canard_amd.solver import Case, CanardError
-    c = Case([cases.periodic_block(8)], {})
+    c = Case([cases.periodic_block(16)], {})
     with pytest.raises(CanardError, match="gridtoy takes"):
-        c.make_grid(0, "toy", 8, 8)
+        c.make_grid(0, "toy", 16, 16)
     with pytest.raises(CanardError, match="not set"):
         c.grid(0)
     c.close()
