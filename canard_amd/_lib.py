@@ -89,6 +89,7 @@ SIGNATURES = [
     ("cnd_net_train", C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_ulonglong, _dp]),
     ("cnd_net_eval", C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     ("cnd_net_apply", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("cnd_net_apply_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     ("cnd_net_forward_host", C.c_int, [C.c_void_p, _fp, _fp]),
     ("cnd_net_grad_host", C.c_int, [C.c_void_p, _fp, _fp, _fp, _fp]),
     ("cnd_box_filter", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),

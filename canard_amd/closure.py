@@ -106,6 +106,10 @@ class Net:
     def apply(self, case, b=0):
         self._check(self.L.cnd_net_apply(self.h, case.h, b), "cnd_net_apply")
 
+    def apply_noalloc(self, case, b=0):
+        """cnd_net_apply with the library's fixed device scratch (n <= 64)."""
+        self._check(self.L.cnd_net_apply_noalloc(self.h, case.h, b), "cnd_net_apply_noalloc")
+
 
 def box_filter(fine, bf, coarse, bc):
     """State of coarse block bc = top-hat filter of fine block bf (include/cnd.h)."""

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -451,6 +452,19 @@ __global__ void k_fill(float *a, float v, int64_t n)
 
 }  // namespace
 
+// Device scratch of cnd_net_apply_noalloc: the network input, the two
+// hidden activations and the output of one apply on an n^3 grid with n up
+// to kScratchN (V = n^3 voxels: BF16 [V][8], [V][32], [V][32], FP32 [V][5],
+// each part aligned to 256 bytes), 41 MiB.  Every GPU the library runs on
+// holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress gives for the current device.
+constexpr int kScratchN = 64;
+constexpr size_t kScratchV = size_t(kScratchN) * kScratchN * kScratchN;
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kScratchBytes = align256(kScratchV * 8 * 2) + 2 * align256(kScratchV * 32 * 2) +
+                                 align256(kScratchV * 5 * 4);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 struct cnd_net {
     int n = 0, V = 0, cap = 0, npairs = 0;
     hipStream_t st = nullptr;
@@ -595,6 +609,32 @@ int net_end(cnd_net *t, hipStream_t st)
     NHIP(t, hipGetLastError());
     NHIP(t, hipEventRecord(t->done, st));
     return CND_OK;
+}
+
+// The address of scratch on the current device, looked up once per device
+// (the first call on it) and kept; nullptr and the error if that fails.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+void *scratch_address(cnd_net *t)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        nfail(t, CND_EDEVICE, "cnd_net_apply_noalloc: no current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            nfail(t, CND_EDEVICE, std::string("cnd_net_apply_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[dev] = ptr;
+    }
+    return g_scratch[dev];
 }
 
 }  // namespace
@@ -844,6 +884,26 @@ int cnd_net_apply(cnd_net *t, cnd_ctx *ctx, int b)
     int r;
     if ((r = coarse_view(t, ctx, b, C, "cnd_net_apply"))) return r;
     return apply_into(t, C, t->x0, t->a1, t->a2, t->y);
+}
+
+int cnd_net_apply_noalloc(cnd_net *t, cnd_ctx *ctx, int b)
+{
+    if (!t) return nfail(t, CND_EINVAL, "cnd_net_apply_noalloc: null network");
+    if (t->n > kScratchN)
+        return nfail(t, CND_EINVAL, "cnd_net_apply_noalloc: grid of " + std::to_string(t->n) +
+                                        "^3 cells exceeds the scratch limit of " + std::to_string(kScratchN) + "^3");
+    CtxView C;
+    int r;
+    if ((r = coarse_view(t, ctx, b, C, "cnd_net_apply_noalloc"))) return r;
+    unsigned char *base = static_cast<unsigned char *>(scratch_address(t));
+    if (!base) return CND_EDEVICE;
+    // every part is written in full before it is read, so nothing is cleared
+    const size_t V = size_t(t->V);
+    bf16 *x0 = reinterpret_cast<bf16 *>(base);
+    bf16 *a1 = reinterpret_cast<bf16 *>(base + align256(V * 8 * 2));
+    bf16 *a2 = reinterpret_cast<bf16 *>(base + align256(V * 8 * 2) + align256(V * 32 * 2));
+    float *y = reinterpret_cast<float *>(base + align256(V * 8 * 2) + 2 * align256(V * 32 * 2));
+    return apply_into(t, C, x0, a1, a2, y);
 }
 
 int cnd_net_forward_host(cnd_net *t, const float *x, float *y)

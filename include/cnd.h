@@ -269,6 +269,15 @@ int  cnd_net_train(cnd_net *net, int iters, double lr, unsigned long long seed, 
 int  cnd_net_eval(cnd_net *net, int first, int count, double *mean_loss);
 /* q += correction(q) on block b of ctx (asynchronous on ctx's stream).     */
 int  cnd_net_apply(cnd_net *net, cnd_ctx *ctx, int b);
+/* The same as cnd_net_apply, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops).  The input, hidden activations and
+ * output live in a fixed device array of the library (41 MiB per GPU, held
+ * until the process ends) instead of the network's own buffers, so n must
+ * be at most 64 (CND_EINVAL otherwise).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (different networks, or one network
+ * on unordered streams) race.                                             */
+int  cnd_net_apply_noalloc(cnd_net *net, cnd_ctx *ctx, int b);
 /* Host test hooks, normalised units, channel-last [n^3][5]: forward, and
  * the gradients of L = sum(gy * y) with respect to x and the parameters.   */
 int  cnd_net_forward_host(cnd_net *net, const float *x, float *y);
