@@ -34,7 +34,10 @@ STEP_BYTES_MODEL = 5048.0  # compulsory FP64 bytes per stored point per viscous 
 # Algorithmic HBM bytes per stored point of one launch of each kernel group
 # (reads + writes of FP64 fields the kernel must touch):
 #   filter sweep (5 variables, one direction): read 5 + write 5 fields
-#   gradient sweep (u,v,w,T along one direction): read q (5) + write 4
+#   gradient sweeps (u,v,w,T, three directions): the xi sweep reads q (5)
+#                   and writes the 4 gradients and the 4 primitives, the
+#                   eta and zeta sweeps read the primitives (4) and write
+#                   4 gradients: 232 B over three launches, 77.33 per launch
 #   stress + flux assembly: read q 5, 12 gradients, 9 metrics + Jacobian;
 #                           write the fluxes of all three directions (15)
 #   flux sweep (one direction): read the 5 fluxes of d, write its own
@@ -44,7 +47,7 @@ STEP_BYTES_MODEL = 5048.0  # compulsory FP64 bytes per stored point per viscous 
 #           as the zeta flux sweep (the "flux" group then has two launches):
 #           read the 5 zeta fluxes, the xi and eta buffers (10), qo 5,
 #           Jacobian; write qa 5 -- the same 208 B
-SECTION_BYTES = {"filter": 80.0, "grad": 72.0, "stress": 336.0, "flux": 80.0, "update": 208.0}
+SECTION_BYTES = {"filter": 80.0, "grad": 232.0 / 3.0, "stress": 336.0, "flux": 80.0, "update": 208.0}
 
 
 def parse():

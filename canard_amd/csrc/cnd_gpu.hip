@@ -989,7 +989,7 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         }
         for (int d = 0; d < 3; ++d)
             for (int k = 0; k < CND_NKIND; ++k) {
-                const int nv = kind_nv(c, k), dflt = d == 0 ? 0 : 2;
+                const int nv = kind_nv(c, k, d), dflt = d == 0 ? 0 : 2;
                 int pick = kTiled;
                 if (!c->tiled_only) {
                     if (shape_sp(c, dflt, d, nv) > 0) pick = dflt;
@@ -1283,13 +1283,13 @@ static int tune(cnd_ctx *c)
             std::vector<int> cands;
             if (!mix && !(kind == CND_KIND_FLUX && d == 2 && fz)) cands.push_back(kTiled);   // no fused tiled form
             for (int sh = 0; sh < kNumShapes; ++sh)
-                if (shape_sp(c, sh, d, kind_nv(c, kind)) > 0) cands.push_back(sh);
+                if (shape_sp(c, sh, d, kind_nv(c, kind, d)) > 0) cands.push_back(sh);
             if (cands.size() < 2) continue;
             int best = c->shape[kind][d];
             double best_ms = 1e300;
             for (int cand : cands) {
                 c->shape[kind][d] = cand;
-                c->seg_sp[kind][d] = cand == kTiled ? 0 : shape_sp(c, cand, d, kind_nv(c, kind));
+                c->seg_sp[kind][d] = cand == kTiled ? 0 : shape_sp(c, cand, d, kind_nv(c, kind, d));
                 constexpr int reps = 3;
                 if ((r = run(kind, d))) break;
                 (void)hipEventRecord(e0, c->st);
@@ -1304,7 +1304,7 @@ static int tune(cnd_ctx *c)
                 if (ms / reps < best_ms) { best_ms = ms / reps; best = cand; }
             }
             c->shape[kind][d] = best;
-            c->seg_sp[kind][d] = best == kTiled ? 0 : shape_sp(c, best, d, kind_nv(c, kind));
+            c->seg_sp[kind][d] = best == kTiled ? 0 : shape_sp(c, best, d, kind_nv(c, kind, d));
             c->tune_ms[kind][d] = best_ms;
         }
     }
@@ -1341,7 +1341,7 @@ int cnd_set_shape(cnd_ctx *c, int kind, int dir, int shape)
     if (c->next_call != 0) return fail(c, CND_ESTATE, "cnd_set_shape: a step is half done");
     if (shape == kTiled && c->precision == CND_MIXED)
         return fail(c, CND_EINVAL, "cnd_set_shape: mixed precision runs on the segmented sweeps only");
-    const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir, kind_nv(c, kind));
+    const int sp = shape == kTiled ? 0 : shape_sp(c, shape, dir, kind_nv(c, kind, dir));
     if (shape != kTiled && sp == 0)
         return fail(c, CND_EINVAL, "cnd_set_shape: the shape does not fit the pencils of this direction");
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }

@@ -422,6 +422,19 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         }
         __syncthreads();
         lu_fetch();
+        if constexpr (Src::kEmit) {
+            // the staged values of the tile, written out point after point
+            if (src.po) {
+                XiWalk xe(t, nthr, n);
+                for (int e = t; e < tot; e += nthr) {
+                    if (xe.p <= xlast) {
+#pragma unroll
+                        for (int m = 0; m < NV; ++m) src.po[m * src.NT + g0 + e] = buf[xe.r * RS + m * P + xe.p];
+                    }
+                    xe.step();
+                }
+            }
+        }
 #pragma unroll
         for (int kr = 0; kr < Sh::KR; ++kr) {
             const int row = a - kb - H + kr;

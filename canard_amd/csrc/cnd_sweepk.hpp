@@ -21,6 +21,7 @@ struct SrcField {   // NV plain fields, vs apart
     int64_t vs;
     struct Raw { double v[NV]; };
     static constexpr bool kDirect = true;   // variable v reads field v only
+    static constexpr bool kEmit = false;    // writes no copy of its values
     __device__ double field(int64_t g, int v) const { return a[v * vs + g]; }
     __device__ void load(int64_t g, int, Raw &r) const
     {
@@ -35,12 +36,16 @@ struct SrcField {   // NV plain fields, vs apart
 };
 
 // u, v, w, T of a state, exactly as the reference's primitives step
-// (src/mo_canard_driver.f90:267-274).
+// (src/mo_canard_driver.f90:267-274).  With po set, xi sweeps also write the
+// values they stage to po (four fields NT apart), for the eta and zeta
+// sweeps to differentiate as plain fields.
 struct SrcPrim {
     const double *q;
     int64_t NT;
+    double *po = nullptr;
     struct Raw { double q[5]; };
     static constexpr bool kDirect = false;
+    static constexpr bool kEmit = true;
     __device__ double field(int64_t, int) const { return 0.0; }
     __device__ void load(int64_t g, int, Raw &r) const
     {
@@ -95,6 +100,7 @@ struct SrcFlux {
     int d;
     struct Raw { double q[5], m[3]; };
     static constexpr bool kDirect = false;
+    static constexpr bool kEmit = false;
     __device__ double field(int64_t, int) const { return 0.0; }
     __device__ void load(int64_t g, int, Raw &r) const
     {
@@ -389,16 +395,17 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
 }
 
 // Velocity and temperature gradients along d; in mixed mode they are stored
-// in FP32.
+// in FP32.  prim (xi sweeps, or null): the primitives u, v, w, T are also
+// written there.
 template <class G, bool MIX>
 __global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX), seg_waves<G>()) void k_seg_grad(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *q, typename std::conditional<MIX, float, double>::type *grad,
-    int64_t NT)
+    int64_t NT, double *prim)
 {
     SEG_PROLOGUE(ntiles)
     using TG = typename std::conditional<MIX, float, double>::type;
-    SrcPrim src{q, NT};
+    SrcPrim src{q, NT, prim};
     SinkOut<4, TG> sink{grad + 4 * d * NT, NT};
     seg_tile<4, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
 }
