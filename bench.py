@@ -148,15 +148,16 @@ def ensemble_compare(steps, warmup, count=32, sizes=(32, 48, 64, 96, 128)):
     n drawn from sizes, Mach and Re drawn per instance), FP64, each step
     replayed as a hipGraph.  Batched: every instance in one context, one
     launch per kernel over all of them.  Separate: one context (and stream)
-    per instance, all queued before any is waited for.  Both are autotuned
-    and warmed up before the timed steps."""
+    per instance, all queued before any is waited for.  Batched mixed: the
+    batched context in mixed precision (FP32 line solves with FP64
+    refinement).  All are autotuned and warmed up before the timed steps."""
     from canard_amd import cases
-    from canard_amd.solver import Case
+    from canard_amd.solver import CND_FP64, CND_MIXED, Case
     blocks, grids, states, params, draws = cases.config4(count=count, sizes=sizes)
     cells = sum(d["n"] ** 3 for d in draws)
 
-    def prepare(blks, grds, sts, prms):
-        c = Case(blks, prms)
+    def prepare(blks, grds, sts, prms, precision=CND_FP64):
+        c = Case(blks, prms, precision=precision)
         c.set_grids(grds)
         c.set_states(sts)
         for i in range(len(prms)):
@@ -188,6 +189,14 @@ def ensemble_compare(steps, warmup, count=32, sizes=(32, 48, 64, 96, 128)):
         a, b = bat.state(i), c.state(0)
         diff = max(diff, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
         c.close()
+    print("[bench] ensemble: batched, mixed precision", file=sys.stderr, flush=True)
+    mix = prepare(blocks, grids, states, params, precision=CND_MIXED)
+    dtm = timed([mix])
+    merr = 0.0
+    for i in range(len(blocks)):
+        a, b = mix.state(i), bat.state(i)
+        merr = max(merr, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
+    mix.close()
     bat.close()
     return {"workload": f"taylor_green_ensemble_{count}x({min(sizes)}..{max(sizes)})^3_periodic_viscous_rk4",
             "instances": len(blocks),
@@ -196,6 +205,8 @@ def ensemble_compare(steps, warmup, count=32, sizes=(32, 48, 64, 96, 128)):
             "separate": {"value": cells * 4 * steps / dts, "ms_per_step": dts / steps * 1e3},
             "speedup_batched_vs_separate": dts / dtb,
             "max_rel_diff_batched_vs_separate": diff,
+            "batched_mixed": {"value": cells * 4 * steps / dtm, "ms_per_step": dtm / steps * 1e3,
+                              "speedup_vs_batched_fp64": dtb / dtm, "max_rel_err_vs_batched_fp64": merr},
             "note": "value in cell-updates/s per RK stage over all instances; the two runs may pick different "
                     "sweep shapes, so their states agree to rounding, not bitwise"}
 
