@@ -12,7 +12,7 @@
 //   forward        out[v][o]  = sum_{t,c} a[nb(v,t)][c] W[o][t][c] (+ b[o])
 //   data gradient  g[v][c]    = sum_{t,o} go[nb(v,t)][o] W[o][26-t][c]
 //   weight grad.   gW[o][t][c] = sum_v go[v][o] a[nb(v,t)][c]   (partial sums
-//                  per voxel chunk, then summed over the chunks in order)
+//                  per workgroup of voxel tiles, then summed in a fixed order)
 // Parameters are FP32 (Adam); BF16 copies in both layouts are rebuilt after
 // every update.  The grid is periodic, n^3 voxels (the context's stored
 // duplicate planes are filled from their originals).
@@ -25,6 +25,7 @@
 #include <mutex>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cnd.h"
@@ -43,7 +44,7 @@ constexpr int kC0 = 5, kC0P = 8, kC1 = 32, kC2 = 32, kC3 = 5, kCP = 32, kTaps = 
 constexpr int kOW1 = 0, kOB1 = kOW1 + kC1 * kTaps * kC0, kOW2 = kOB1 + kC1, kOB2 = kOW2 + kC2 * kTaps * kC1,
               kOW3 = kOB2 + kC2, kOB3 = kOW3 + kC3 * kTaps * kC2, kNParam = kOB3 + kC3;
 constexpr int kM1 = kTaps * kC0P, kM2 = kTaps * kCP;   // weight-gradient row lengths
-constexpr int kChunk = 2048;                           // voxels per wave (chunk) in the weight gradient
+constexpr int kWgSlices = 8;                           // slices of the first weight-gradient reduction level
 
 __host__ __device__ inline uint32_t f2u(float f)
 {
@@ -193,60 +194,156 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv(const bf16 *__restrict
     }
 }
 
-// Partial weight gradients of one voxel chunk: part[chunk][o][m] = sum over
-// the chunk's voxels v of go[v][o] a[nb(v, m / CIP)][m % CIP], on
-// v_mfma_f32_32x32x16_bf16 (K = 16 voxels per instruction); grid (ceil(27
-// CIP / 32), ceil(V / (4 kChunk))), four waves per workgroup on separate
-// chunks.  k_grad_reduce sums the chunks (a fixed order: no atomics, the
-// gradient is the same on every run).
-template <int CIP>
-__global__ __launch_bounds__(256) void k_wgrad(const bf16 *__restrict__ go, const bf16 *__restrict__ act,
-                                               float *__restrict__ part, int n)
+// Weight and bias gradients of one layer as a GEMM over voxels on
+// v_mfma_f32_32x32x16_bf16: dW[o][m] = sum_v go[v][o] a[nb(v, m / CIP)][m % CIP]
+// (M = 27 CIP columns, in 32-column tiles) and db[o] = sum_v go[v][o].
+// Workgroups of 8 waves walk the 256-voxel tiles of k_conv (tile =
+// blockIdx.x, blockIdx.x + gridDim.x, ...).  Per tile they stage the
+// activation halo (VS BF16 per voxel: the two lane halves, 8 voxels apart,
+// then read different LDS banks) and the output gradient transposed,
+// goT[o][vl] with vl = lx + X (ly + TY lz), in LDS.  Wave w accumulates the
+// column tiles w, w + 8, ... over all voxels of its workgroup, 16 per
+// instruction: the A operand (lane: o = lane % 32, 8 consecutive voxels
+// along x) is one 16-byte LDS read shared by all of the wave's columns, the
+// B operand the 8 neighbour values of the lane's (tap, channel).  Wave 7
+// also sums the A operands for the bias gradient.  The workgroup's sums go
+// to part[block][o][m] and bpart[block][o]; k_grad_part and k_grad_reduce
+// add the blocks in a fixed order (no atomics: the gradient is the same on
+// every run).
+constexpr int kWgGS = 256 + 8;   // goT row stride (voxels)
+constexpr int kWgBlocks = 256;   // workgroups of a weight-gradient launch (at most)
+__host__ __device__ constexpr int wg_vs(int cip) { return cip + 4; }
+__host__ inline size_t wgrad_lds_bytes(int cip, int n)
 {
-    const int V = n * n * n, M = kTaps * CIP;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = lane & 31, kh = lane >> 5;
-    const int chunk = blockIdx.y * 4 + wave;
-    const int vb0 = chunk * kChunk;
-    if (vb0 >= V) return;
-    const int vend = min(V, vb0 + kChunk);
-    const int m = blockIdx.x * 32 + i;
-    const bool mok = m < M;
-    const int t = mok ? m / CIP : 0, c = mok ? m % CIP : 0;
-    f16v acc;
+    const int X = conv_x(n), R = 32 / X;
+    return sizeof(bf16) * (size_t(32) * kWgGS + size_t(X + 2) * (2 * R + 2) * 6 * wg_vs(cip));
+}
+__host__ inline int wgrad_blocks(int n)
+{
+    const int tiles = n * n * n / 256;
+    return tiles < kWgBlocks ? tiles : kWgBlocks;
+}
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+template <int CIP>
+__global__ __launch_bounds__(512) void k_wgrad(const bf16 *__restrict__ go, const bf16 *__restrict__ act,
+                                               float *__restrict__ part, float *__restrict__ bpart, int n)
+{
+    constexpr int VS = wg_vs(CIP), M = kTaps * CIP, NT = (M + 31) / 32, TPW = (NT + 7) / 8;
+    extern __shared__ unsigned short wg_lds[];
+    bf16 *goT = wg_lds;               // [32][kWgGS]
+    bf16 *sa = wg_lds + 32 * kWgGS;   // halo [HZ][HY][HX][VS]
+    const int X = conv_x(n), R = 32 / X;
+    const int TY = 2 * R, HX = X + 2, HY = TY + 2, HZ = 6, HV = HX * HY * HZ;
+    const int ntx = n / X, nty = n / TY;
+    const int ntiles = ntx * nty * (n / 4);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int wave = tid >> 6, lane = tid & 63, i = lane & 31, kh = lane >> 5;
+    // the wave's column tiles j = wave + 8 u: the lane's column m = 32 j + i,
+    // its tap's halo offset and its channel
+    int toff[TPW], coff[TPW];
+    bool jok[TPW], mok[TPW];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    for (int vb = vb0; vb < vend; vb += 16) {   // V and kChunk are multiples of 16 (n a multiple of 8)
-        s8v a, b;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int v = vb + 8 * kh + e;
-            a[e] = short(go[int64_t(v) * kCP + i]);
-            const int x = v % n, y = (v / n) % n, z = v / (n * n);
-            b[e] = mok ? short(act[int64_t(nbr(x, y, z, t, n)) * CIP + c]) : short(0);
-        }
-        acc = mfma16(a, b, acc);
+    for (int u = 0; u < TPW; ++u) {
+        const int j = wave + 8 * u, m = 32 * j + i;
+        jok[u] = j < NT;
+        mok[u] = jok[u] && m < M;
+        const int t = mok[u] ? m / CIP : 0;
+        toff[u] = ((t / 9) * HY + (t / 3) % 3) * HX + t % 3;
+        coff[u] = mok[u] ? m % CIP : 0;
     }
-    if (!mok) return;
+    f16v acc[TPW];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int o = 8 * (r >> 2) + 4 * kh + (r & 3);
-        part[(int64_t(chunk) * kCP + o) * M + m] = acc[r];
+    for (int u = 0; u < TPW; ++u)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[u][r] = 0.f;
+    float bsum = 0.f;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int w = tile;
+        const int x0 = (w % ntx) * X;
+        w /= ntx;
+        const int y0 = (w % nty) * TY, z0 = (w / nty) * 4;
+        __syncthreads();   // the previous tile's operands are read
+        for (int e = tid; e < HV * (CIP / 4); e += nthr) {
+            const int hv = e / (CIP / 4), c4 = e - hv * (CIP / 4);
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int64_t gv = wrap(x0 + hx - 1, n) + int64_t(n) * (wrap(y0 + hy - 1, n) + int64_t(n) * wrap(z0 + hz - 1, n));
+            *reinterpret_cast<s4v *>(sa + hv * VS + 4 * c4) = *reinterpret_cast<const s4v *>(act + gv * CIP + 4 * c4);
+        }
+        for (int e = tid; e < 256 * (kCP / 8); e += nthr) {
+            const int vl = e / (kCP / 8), c8 = e - vl * (kCP / 8);
+            const int lx = vl % X, ly = (vl / X) % TY, lz = vl / (X * TY);
+            const int64_t v = (x0 + lx) + int64_t(n) * ((y0 + ly) + int64_t(n) * (z0 + lz));
+            const s8v g = *reinterpret_cast<const s8v *>(go + v * kCP + 8 * c8);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) goT[(8 * c8 + k) * kWgGS + vl] = bf16(g[k]);
+        }
+        __syncthreads();
+        for (int q = 0; q < 16; ++q) {
+            const int vl0 = 16 * q + 8 * kh;   // the lane's 8 voxels: vl0 .. vl0 + 7, along x
+            const s8v a = *reinterpret_cast<const s8v *>(goT + i * kWgGS + vl0);
+            if (wave == 7) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bsum += from_bf16(bf16(a[e]));
+            }
+            const int lx = vl0 % X, ly = (vl0 / X) % TY, lz = vl0 / (X * TY);
+            const int hb = (lz * HY + ly) * HX + lx;
+#pragma unroll
+            for (int u = 0; u < TPW; ++u) {
+                if (!jok[u]) continue;
+                const bf16 *bp = sa + (hb + toff[u]) * VS + coff[u];
+                s8v bv;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bv[e] = mok[u] ? short(bp[e * VS]) : short(0);
+                acc[u] = mfma16(a, bv, acc[u]);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        if (!mok[u]) continue;
+        const int m = 32 * (wave + 8 * u) + i;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int o = 8 * (r >> 2) + 4 * kh + (r & 3);
+            part[(int64_t(blockIdx.x) * kCP + o) * M + m] = acc[u][r];
+        }
+    }
+    if (wave == 7) {
+        bsum += __shfl_xor(bsum, 32);
+        if (kh == 0) bpart[blockIdx.x * kCP + i] = bsum;
     }
 }
 
-// Partial bias gradients: part[chunk][o] = sum over 256 voxels of go[v][o];
-// one thread per (chunk, channel).
-__global__ __launch_bounds__(256) void k_bgrad(const bf16 *__restrict__ go, float *__restrict__ part, int V)
+// Bias gradient: gb[o] = sum of part[c][o] over the nch blocks c; one
+// workgroup, eight slices of blocks summed in order.
+__global__ __launch_bounds__(256) void k_bias_reduce(const float *__restrict__ part, float *__restrict__ gb, int nch)
 {
-    const int o = threadIdx.x & 31;
-    const int ch = blockIdx.x * 8 + (threadIdx.x >> 5);
-    const int v0 = ch * 256;
-    if (v0 >= V) return;
-    const int v1 = min(V, v0 + 256);
+    __shared__ float sl[8][kCP];
+    const int o = threadIdx.x & (kCP - 1), k = threadIdx.x / kCP;
     float s = 0.f;
-    for (int v = v0; v < v1; ++v) s += from_bf16(go[int64_t(v) * kCP + o]);
-    part[int64_t(ch) * kCP + o] = s;
+    for (int c = k; c < nch; c += 8) s += part[c * kCP + o];
+    sl[k][o] = s;
+    __syncthreads();
+    if (threadIdx.x < kCP) {
+        float t = 0.f;
+        for (int j = 0; j < 8; ++j) t += sl[j][o];
+        gb[o] = t;
+    }
+}
+
+// First level of the block reductions: out[s][k] = sum of part[c][k] over
+// blocks c = s per .. (s + 1) per - 1 (below nch), k < len; grid (len / 256,
+// slices).
+__global__ __launch_bounds__(256) void k_grad_part(const float *__restrict__ part, float *__restrict__ out, int len,
+                                                   int nch, int per)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= len) return;
+    const int c0 = blockIdx.y * per, c1 = min(nch, c0 + per);
+    float s = 0.f;
+    for (int c = c0; c < c1; ++c) s += part[int64_t(c) * len + k];
+    out[int64_t(blockIdx.y) * len + k] = s;
 }
 
 // g[k] = sum over nch chunks of part[chunk][k], k < len (second level of the
@@ -262,26 +359,54 @@ __global__ __launch_bounds__(256) void k_grad_reduce(const float *__restrict__ p
 }
 
 // Mean-square loss over V x 5 values: g = 2 (y - t) / (5 V) as BF16 [V][32]
-// (channels 5.. zero), loss += sum (y - t)^2.
+// (channels 5.. zero), and each workgroup's sum of (y - t)^2 in
+// lpart[block] (k_loss_add adds them to the running loss in order).
 __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ y, const float *__restrict__ tg,
-                                             bf16 *__restrict__ g, double *__restrict__ loss, int V)
+                                             bf16 *__restrict__ g, double *__restrict__ lpart, int V)
 {
+    __shared__ double ws[4];
     const int v = blockIdx.x * 256 + threadIdx.x;
     double s = 0.0;
     if (v < V) {
         const float sc = 2.0f / (5.0f * float(V));
-        for (int c = 0; c < kCP; ++c) {
+        s8v g0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
             float gv = 0.f;
             if (c < kC3) {
                 const float d = y[int64_t(v) * kC3 + c] - tg[int64_t(v) * kC3 + c];
                 s += double(d) * double(d);
                 gv = sc * d;
             }
-            g[int64_t(v) * kCP + c] = to_bf16(gv);
+            g0[c] = short(to_bf16(gv));
         }
+        s8v z;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) z[c] = 0;
+        s8v *gp = reinterpret_cast<s8v *>(g + int64_t(v) * kCP);
+        gp[0] = g0;
+#pragma unroll
+        for (int k = 1; k < kCP / 8; ++k) gp[k] = z;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, s);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) lpart[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+// loss += the sum of lpart[0 .. nb - 1] (one workgroup, slices summed in order).
+__global__ __launch_bounds__(256) void k_loss_add(const double *__restrict__ lpart, double *__restrict__ loss, int nb)
+{
+    __shared__ double sl[256];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) s += lpart[b];
+    sl[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) sl[threadIdx.x] += sl[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss += sl[0];
 }
 
 // BF16 [V][32] from an FP32 [V][5] gradient (test hook).
@@ -475,11 +600,12 @@ struct cnd_net {
     Norm nm{};
     float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;   // parameters, gradient, Adam moments
     float *gacc = nullptr;                                            // gw1 | gw2 | gw3 | gb (3 x 32)
-    float *gpart = nullptr, *bpart = nullptr;   // per-chunk partial weight and bias gradients
+    // weight and bias gradients per workgroup, and per slice of workgroups
+    float *gpart = nullptr, *gpart2 = nullptr, *bpart = nullptr;
     Wts W{};
     bf16 *x0 = nullptr, *a1 = nullptr, *a2 = nullptr, *g3 = nullptr, *g2 = nullptr, *g1 = nullptr;
     float *y = nullptr, *gx = nullptr, *hx = nullptr;
-    double *loss = nullptr;
+    double *loss = nullptr, *lpart = nullptr;   // running loss, per-workgroup sums of one pair
     bf16 *din = nullptr;    // pairs: inputs [cap][V][8]
     float *dtg = nullptr;   // pairs: targets [cap][V][5]
     long long adam_t = 0;
@@ -558,30 +684,31 @@ int backward(cnd_net *t, const bf16 *x0, float *gx, hipStream_t st)
     const int V = t->V;
     const dim3 b(256), gv(unsigned(V / (32 * kConvWaves))), bc(64 * kConvWaves);
     const size_t l32 = conv_lds_bytes(kCP, t->n);
-    const unsigned nchw = blocks_for(V, 4 * kChunk);   // workgroups of four chunks
-    const int nch = (V + kChunk - 1) / kChunk, nbch = (V + 255) / 256;
-    const unsigned nbg = blocks_for(nbch, 8);
-    auto wgrad_sum = [&](float *gw, int M) {
-        hipLaunchKernelGGL(k_grad_reduce, dim3(blocks_for(int64_t(kCP) * M, 256)), b, 0, st, t->gpart, gw, kCP * M, nch);
+    const int G = wgrad_blocks(t->n);
+    // weight and bias gradients of a layer from its output gradient go and
+    // input activations act (CIP channels): G block partials, then
+    // kWgSlices slice sums, then the total
+    auto wgrad = [&](auto cip, const bf16 *go, const bf16 *act, float *gw, float *gb) {
+        constexpr int CIP = decltype(cip)::value, M = kTaps * CIP;
+        hipLaunchKernelGGL(k_wgrad<CIP>, dim3(unsigned(G)), dim3(512), wgrad_lds_bytes(CIP, t->n), st, go, act,
+                           t->gpart, t->bpart, t->n);
+        const int per = (G + kWgSlices - 1) / kWgSlices;
+        hipLaunchKernelGGL(k_grad_part, dim3(blocks_for(int64_t(kCP) * M, 256), kWgSlices), b, 0, st, t->gpart,
+                           t->gpart2, kCP * M, G, per);
+        hipLaunchKernelGGL(k_grad_reduce, dim3(blocks_for(int64_t(kCP) * M, 256)), b, 0, st, t->gpart2, gw, kCP * M,
+                           kWgSlices);
+        hipLaunchKernelGGL(k_bias_reduce, dim3(1), b, 0, st, t->bpart, gb, G);
     };
-    auto bgrad = [&](const bf16 *go, float *gb) {
-        hipLaunchKernelGGL(k_bgrad, dim3(nbg), b, 0, st, go, t->bpart, V);
-        hipLaunchKernelGGL(k_grad_reduce, dim3(1), b, 0, st, t->bpart, gb, kCP, nbch);
-    };
+    using C32 = std::integral_constant<int, kCP>;
+    using C8 = std::integral_constant<int, kC0P>;
     // layer 3
-    hipLaunchKernelGGL(k_wgrad<kCP>, dim3(kM2 / 32, nchw), b, 0, st, t->g3, t->a2, t->gpart, t->n);
-    wgrad_sum(gw3(t), kM2);
-    bgrad(t->g3, gbias(t) + 2 * kCP);
+    wgrad(C32{}, t->g3, t->a2, gw3(t), gbias(t) + 2 * kCP);
     hipLaunchKernelGGL((k_conv<kCP, 2>), gv, bc, l32, st, t->g3, t->W.wd3, nullptr, t->a2, t->g2, t->n, kCP, kCP);
     // layer 2
-    hipLaunchKernelGGL(k_wgrad<kCP>, dim3(kM2 / 32, nchw), b, 0, st, t->g2, t->a1, t->gpart, t->n);
-    wgrad_sum(gw2(t), kM2);
-    bgrad(t->g2, gbias(t) + kCP);
+    wgrad(C32{}, t->g2, t->a1, gw2(t), gbias(t) + kCP);
     hipLaunchKernelGGL((k_conv<kCP, 2>), gv, bc, l32, st, t->g2, t->W.wd2, nullptr, t->a1, t->g1, t->n, kCP, kCP);
     // layer 1
-    hipLaunchKernelGGL(k_wgrad<kC0P>, dim3((kM1 + 31) / 32, nchw), b, 0, st, t->g1, x0, t->gpart, t->n);
-    wgrad_sum(gw1(t), kM1);
-    bgrad(t->g1, gbias(t));
+    wgrad(C8{}, t->g1, x0, gw1(t), gbias(t));
     if (gx)
         hipLaunchKernelGGL((k_conv<kCP, 1>), gv, bc, l32, st, t->g1, t->W.wd1, nullptr, nullptr, gx, t->n, kC0, kC0);
     hipLaunchKernelGGL(k_gather_grad, dim3(blocks_for(kNParam, 256)), b, 0, st, gw1(t), gw2(t), gw3(t), gbias(t),
@@ -658,10 +785,10 @@ int cnd_net_create(int n, int capacity, unsigned long long seed, cnd_net **out)
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
             return nfail(nullptr, CND_EDEVICE, "cnd_net_create: device attributes");
-        if (conv_lds_bytes(kCP, n) > size_t(lds))
-            return nfail(nullptr, CND_EDEVICE, "cnd_net_create: the convolution tiles need " +
-                                                   std::to_string(conv_lds_bytes(kCP, n)) + " bytes of LDS, the device has " +
-                                                   std::to_string(lds));
+        const size_t need = std::max(conv_lds_bytes(kCP, n), wgrad_lds_bytes(kCP, n));
+        if (need > size_t(lds))
+            return nfail(nullptr, CND_EDEVICE, "cnd_net_create: the convolution tiles need " + std::to_string(need) +
+                                                   " bytes of LDS, the device has " + std::to_string(lds));
     }
     cnd_net *t = new cnd_net;
     t->n = n;
@@ -676,14 +803,16 @@ int cnd_net_create(int n, int capacity, unsigned long long seed, cnd_net **out)
     if (!good) r = nfail(t, CND_EDEVICE, "stream or event creation failed");
     good = good && ok(nalloc(t, &t->p, kNParam)) && ok(nalloc(t, &t->g, kNParam)) && ok(nalloc(t, &t->m1, kNParam)) &&
            ok(nalloc(t, &t->m2, kNParam)) && ok(nalloc(t, &t->gacc, kGacc)) && ok(nalloc(t, &t->W.wf1, S8)) &&
-           ok(nalloc(t, &t->gpart, int64_t((V + kChunk - 1) / kChunk) * kCP * kM2)) &&
-           ok(nalloc(t, &t->bpart, int64_t((V + 255) / 256) * kCP)) &&
+           ok(nalloc(t, &t->gpart, int64_t(wgrad_blocks(n)) * kCP * kM2)) &&
+           ok(nalloc(t, &t->gpart2, int64_t(kWgSlices) * kCP * kM2)) &&
+           ok(nalloc(t, &t->bpart, int64_t(wgrad_blocks(n)) * kCP)) &&
            ok(nalloc(t, &t->W.wf2, S32)) && ok(nalloc(t, &t->W.wf3, S32)) && ok(nalloc(t, &t->W.wd1, S32)) &&
            ok(nalloc(t, &t->W.wd2, S32)) && ok(nalloc(t, &t->W.wd3, S32)) && ok(nalloc(t, &t->W.bb1, kCP)) &&
            ok(nalloc(t, &t->W.bb2, kCP)) && ok(nalloc(t, &t->W.bb3, kCP)) && ok(nalloc(t, &t->x0, V * kC0P)) &&
            ok(nalloc(t, &t->a1, V * kCP)) && ok(nalloc(t, &t->a2, V * kCP)) && ok(nalloc(t, &t->g3, V * kCP)) &&
            ok(nalloc(t, &t->g2, V * kCP)) && ok(nalloc(t, &t->g1, V * kCP)) && ok(nalloc(t, &t->y, V * kC3)) &&
            ok(nalloc(t, &t->gx, V * kC0)) && ok(nalloc(t, &t->hx, V * kC0)) && ok(nalloc(t, &t->loss, 1)) &&
+           ok(nalloc(t, &t->lpart, (V + 255) / 256)) &&
            ok(nalloc(t, &t->din, int64_t(capacity) * V * kC0P)) && ok(nalloc(t, &t->dtg, int64_t(capacity) * V * kC3));
     if (!good) {
         std::string m = t->err;
@@ -719,9 +848,9 @@ void cnd_net_destroy(cnd_net *t)
     if (!t) return;
     if (t->done) (void)hipEventSynchronize(t->done);
     if (t->st) (void)hipStreamSynchronize(t->st);
-    void *ptrs[] = {t->p, t->g, t->m1, t->m2, t->gacc, t->gpart, t->bpart, t->W.wf1, t->W.wf2, t->W.wf3, t->W.wd1, t->W.wd2, t->W.wd3,
+    void *ptrs[] = {t->p, t->g, t->m1, t->m2, t->gacc, t->gpart, t->gpart2, t->bpart, t->W.wf1, t->W.wf2, t->W.wf3, t->W.wd1, t->W.wd2, t->W.wd3,
                     t->W.bb1, t->W.bb2, t->W.bb3, t->x0, t->a1, t->a2, t->g3, t->g2, t->g1, t->y, t->gx, t->hx,
-                    t->loss, t->din, t->dtg};
+                    t->loss, t->lpart, t->din, t->dtg};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (t->done) (void)hipEventDestroy(t->done);
     if (t->st) (void)hipStreamDestroy(t->st);
@@ -839,7 +968,8 @@ int cnd_net_train(cnd_net *t, int iters, double lr, unsigned long long seed, dou
         const bf16 *x0 = t->din + int64_t(k) * V * kC0P;
         if ((r = forward(t, x0, st))) return r;
         hipLaunchKernelGGL(k_mse, dim3(blocks_for(V, 256)), dim3(256), 0, st, t->y, t->dtg + int64_t(k) * V * kC3,
-                           t->g3, t->loss, int(V));
+                           t->g3, t->lpart, int(V));
+        hipLaunchKernelGGL(k_loss_add, dim3(1), dim3(256), 0, st, t->lpart, t->loss, int(blocks_for(V, 256)));
         if ((r = backward(t, x0, nullptr, st))) return r;
         ++t->adam_t;
         const float c1 = 1.f - std::pow(0.9f, float(t->adam_t)), c2 = 1.f - std::pow(0.999f, float(t->adam_t));
@@ -867,7 +997,8 @@ int cnd_net_eval(cnd_net *t, int first, int count, double *mean_loss)
     for (int k = first; k < first + count; ++k) {
         if ((r = forward(t, t->din + int64_t(k) * V * kC0P, st))) return r;
         hipLaunchKernelGGL(k_mse, dim3(blocks_for(V, 256)), dim3(256), 0, st, t->y, t->dtg + int64_t(k) * V * kC3,
-                           t->g3, t->loss, int(V));
+                           t->g3, t->lpart, int(V));
+        hipLaunchKernelGGL(k_loss_add, dim3(1), dim3(256), 0, st, t->lpart, t->loss, int(blocks_for(V, 256)));
     }
     double s = 0.0;
     NHIP(t, hipMemcpyAsync(&s, t->loss, sizeof(double), hipMemcpyDeviceToHost, st));
