@@ -198,11 +198,11 @@ inline bool seg_on(cnd_ctx *c, int kind, int d) { return c->shape[kind][d] != kT
 
 
 // Variables per workgroup of a sweep kind along d (viscous flux sweeps run
-// one workgroup column per component of the stored fluxes; in FP64 the eta
-// and zeta gradient sweeps, one per primitive the xi sweep stored).
+// one workgroup column per component of the stored fluxes; the eta and zeta
+// gradient sweeps, one per primitive the xi sweep stored).
 inline int kind_nv(const cnd_ctx *c, int kind, int d)
 {
-    if (kind == CND_KIND_GRAD) return d > 0 && c->precision != CND_MIXED ? 1 : 4;
+    if (kind == CND_KIND_GRAD) return d > 0 ? 1 : 4;
     return kind == CND_KIND_FLUX && !c->any_visc ? 5 : 1;
 }
 
@@ -222,10 +222,10 @@ inline bool fuse_possible(const cnd_ctx *c)
 int run_filter_field(cnd_ctx *c, int d, const double *in, double *out, int nvar, bool mix = false);
 int run_fd_field(cnd_ctx *c, int d, int n45, const double *in, int64_t ivs, double *out, int64_t ovs, int nvar,
                  bool mix = false);
-// Gradient sweep along d of the state cur (kind CND_KIND_GRAD).  In FP64
-// the xi sweep (d = 0) also stores the primitives u, v, w, T in the flux
-// scratch fl, and the eta and zeta sweeps (segmented) differentiate those:
-// the three must run in order, d = 0 first.
+// Gradient sweep along d of the state cur (kind CND_KIND_GRAD).  The xi
+// sweep (d = 0) also stores the primitives u, v, w, T in the flux scratch
+// fl, and the eta and zeta sweeps (segmented) differentiate those: the
+// three must run in order, d = 0 first.
 int run_grad(cnd_ctx *c, int d, const double *cur);
 // Flux sweep along d into the d-th right-hand-side buffer (kind
 // CND_KIND_FLUX).

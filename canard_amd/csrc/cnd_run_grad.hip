@@ -31,11 +31,12 @@ __global__ __launch_bounds__(kPts) void k_prims(const BlkDev *__restrict__ blks,
 int run_grad(cnd_ctx *c, int d, const double *cur)
 {
     const int K = CND_KIND_GRAD;
+    const bool mix = c->precision == CND_MIXED;   // FP32 solves and gradients (segmented only)
     double *prim = c->fl;   // the fluxes of the previous stage are consumed by now
-    if (c->precision == CND_MIXED) {
-        LAUNCH_SEG(k_seg_grad, true, 4, c, K, d, 1, cur, c->gradf, c->NT, (double *)nullptr);
-    } else if (d == 0) {
-        if (seg_on(c, K, 0)) {
+    if (d == 0) {
+        if (mix) {
+            LAUNCH_SEG(k_seg_grad, true, 4, c, K, 0, 1, cur, c->gradf, c->NT, prim);
+        } else if (seg_on(c, K, 0)) {
             LAUNCH_SEG(k_seg_grad, false, 4, c, K, 0, 1, cur, c->grad, c->NT, prim);
         } else {
             LAUNCH_SWEEP(k_fd_grad, 4, c, 0, 1, 4 * kTile, c->dblk, c->dtab, c->dlu, c->dtiles[0], ntiles(c, 0), 0,
@@ -43,8 +44,10 @@ int run_grad(cnd_ctx *c, int d, const double *cur)
             if (seg_on(c, K, 1) || seg_on(c, K, 2))
                 hipLaunchKernelGGL(k_prims, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dpt, cur, prim, c->NT);
         }
+    } else if (mix) {
+        LAUNCH_SEG(k_seg_grad_prim, true, 1, c, K, d, 4, prim, c->gradf, c->NT);
     } else if (seg_on(c, K, d)) {
-        LAUNCH_SEG(k_seg_fd_field, false, 1, c, K, d, 4, 0, prim, c->NT, c->grad + 4 * d * c->NT, c->NT);
+        LAUNCH_SEG(k_seg_grad_prim, false, 1, c, K, d, 4, prim, c->grad, c->NT);
     } else {
         LAUNCH_SWEEP(k_fd_grad, 4, c, d, 1, 4 * kTile, c->dblk, c->dtab, c->dlu, c->dtiles[d], ntiles(c, d), d,
                      c->nmax[d], c->ys, cur, c->grad, c->NT);

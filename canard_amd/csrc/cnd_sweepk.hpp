@@ -410,6 +410,23 @@ __global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX), seg_waves<G>()) void
     seg_tile<4, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
 }
 
+// Gradients along d (eta, zeta) of the primitive v = blockIdx.y that the xi
+// gradient sweep stored (prim, four fields NT apart), into grad; in mixed
+// mode FP32.
+template <class G, bool MIX>
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_grad_prim(
+    const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
+    int ntiles, int SP, int d, const double *prim, typename std::conditional<MIX, float, double>::type *grad,
+    int64_t NT)
+{
+    SEG_PROLOGUE(ntiles)
+    using TG = typename std::conditional<MIX, float, double>::type;
+    const int v = blockIdx.y;
+    SrcField<1> src{prim + v * NT, 0};
+    SinkOut<1, TG> sink{grad + (4 * d + v) * NT, 0};
+    seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, tiles[tix], d, 0, SP, src, sink, lds);
+}
+
 template <class G, bool MIX>
 __global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,

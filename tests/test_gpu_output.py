@@ -65,6 +65,26 @@ def test_output_schedule_and_files(tmp_path):
     c.close()
 
 
+def test_output_schedule_under_graph_replay():
+    """The schedule is decided on the device inside the step, so steps
+    replayed as a graph report the same outputs as eager steps."""
+    a, _ = _case()
+    b, _ = _case()
+    b.use_graph(True)
+    for c in (a, b):
+        c.output_schedule(1.0, 8)
+    seq = {0: [], 1: []}
+    for _ in range(40):
+        for k, c in enumerate((a, b)):
+            c.step(1)
+            seq[k].append(c.output_due())
+    print("[output graph]", sum(d for d, _, _ in seq[0]), "outputs in 40 steps", flush=True)
+    assert seq[0] == seq[1]
+    assert sum(d for d, _, _ in seq[0]) >= 2
+    for c in (a, b):
+        c.close()
+
+
 def test_async_output_equals_sync(tmp_path):
     a, _ = _case(24)
     b, _ = _case(24)
