@@ -97,6 +97,29 @@ def test_tiled_xi_gradients_with_segmented_eta_zeta():
     o.close()
 
 
+@pytest.mark.parametrize("nxyz", [(600, 16, 14), (2111, 12, 12)])
+def test_long_xi_pencils_fall_back_to_tiled(nxyz):
+    """Pencils of 601 and 2112 points (the largest a block may have) along
+    xi fit no segmented xi shape (528 at most): every xi sweep runs tiled,
+    the primitives for the segmented eta and zeta gradient sweeps come from
+    the pointwise kernel; the second case also has the fewest points a
+    direction may have (13) along eta and zeta."""
+    blocks, grids, states, params = tgv_case(0, nxyz=nxyz, pert=1e-3)
+    c = _case(blocks, grids, states, params)
+    t = c.tuning()
+    print("[long xi]", {k: [r["shape"] for r in v] for k, v in t.items()}, flush=True)
+    assert all(t[k][0]["shape"] == -1 for k in ("filter", "grad", "flux"))
+    assert t["grad"][1]["shape"] >= 0 and t["grad"][2]["shape"] >= 0
+    o = oracle_for(blocks, grids, states, params)
+    c.step(5)
+    o.run(5)
+    err = rel_err(c.state(0), o.state(0))
+    print("  rel err per variable", err)
+    assert np.all(err < 1e-12), err
+    c.close()
+    o.close()
+
+
 def test_autotuned_step_matches_oracle():
     blocks, grids, states, params = tgv_case(40, pert=1e-3)
     c = _case(blocks, grids, states, params)
