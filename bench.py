@@ -214,8 +214,8 @@ def ensemble_compare(steps, warmup, count=32, sizes=(32, 48, 64, 96, 128)):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the per-block step does not shard "
-              "across GPUs in this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the library's step runs on one GPU per "
+              "process and does not shard across GPUs.", file=sys.stderr)
         sys.exit(2)
     from canard_amd.solver import CND_FP64, CND_MIXED, Case
 

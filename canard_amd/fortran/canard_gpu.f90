@@ -8,7 +8,7 @@
 !
 ! Namelists: nml_driver, nml_canard, nml_domdcomp, nml_numerics,
 ! nml_physics, nml_gridgen, nml_sponge as in the reference (README.md:44-99),
-! plus nml_gpu for this build's switches:
+! plus nml_gpu for the switches of the GPU library:
 !   viscous   (1)  Navier-Stokes (the reference's -DVISCOUS build) or Euler (0)
 !   precision (0)  0 FP64, 1 FP32 sweeps with FP64 refinement
 !   maxsteps  (-1) stop after this many steps as well (-1: no cap)

@@ -1,5 +1,6 @@
 """bench.py's command line on the CPU: more than one GPU is refused with a
-message (one GPU per run in this build), and the options parse."""
+message (the library's step runs on one GPU per process), and the options
+parse."""
 import os
 import subprocess
 import sys

@@ -165,7 +165,9 @@ def test_one_sided_rows_nearly_exact_on_polynomials():
 
 def test_junction_partner_counts():
     """Triply periodic single block: every corner has 7 junction partners and
-    every edge 3 (the survey's restatement of search_point/search_line)."""
+    every edge 3, as search_point/search_line (src/mo_domdcomp.f90:269-437)
+    find them when a block is its own partner across every face (a corner
+    point appears in 8 places of the periodic lattice, an edge point in 4)."""
     o = orc.Oracle([per_block(16)])
     p8, l12 = o.junctions(0)
     assert p8 == [7] * 8 and l12 == [3] * 12
