@@ -162,6 +162,37 @@ def test_multiblock_gcbc_sponge_matches_oracle(sweep_mode):
     o.close()
 
 
+@pytest.mark.parametrize("case", ["tgv32", "gcbc_sponge_pair"])
+def test_200_steps_match_oracle(case):
+    """Long horizon: 200 steps (dt recomputed 20 times) of a 32^3 TGV and of
+    the sheared two-block GCBC + sponge case, state within 1e-12 of the
+    oracle per conserved variable; the error of the change is printed."""
+    if case == "tgv32":
+        blocks, grids, states, params = tgv_case(32, pert=1e-3)
+    else:
+        sizes = [(24, 16, 14), (16, 16, 14)]
+        blocks = cases.sheared_pair_blocks(sizes)
+        grids = cases.sheared_pair_grids(sizes, h=1.0, amp=0.05)
+        states = [cases.pulse_state(g, mach=0.5, centres=[(20.0, 8.0, 7.0)], amp=1e-3, width=3.0) for g in grids]
+        params = dict(reoo=200.0, amach1=0.5, nsmf=1, cfl=0.8, sponge=1, szco=0.5, szth0=6.0, szth1=6.0,
+                      doml0=0.0, doml1=40.0)
+    print(f"[long] {case}, 200 steps", flush=True)
+    c = _case(blocks, grids, states, params)
+    o = oracle_for(blocks, grids, states, params, threads=8)
+    c.step(200)
+    o.run(200)
+    for b in range(len(blocks)):
+        err = rel_err(c.state(b), o.state(b))
+        chg = rel_err(c.state(b) - states[b], o.state(b) - states[b])
+        print(f"  block {b}: rel err {err}, of the change {chg}", flush=True)
+        assert np.all(err < 1e-12), (b, err)
+    cg, co = c.clock(), o.clock()
+    assert cg["n"] == co["n"] == 200
+    assert abs(cg["timo"] - co["timo"]) <= 1e-13 * co["timo"]
+    c.close()
+    o.close()
+
+
 def test_inviscid_steps_match_oracle(sweep_mode):
     """Euler steps (viscous = 0): the fluxes are evaluated inside the flux
     sweeps instead of being assembled with the stresses.  A TGV block and
