@@ -715,7 +715,7 @@ size_t seg_lds_bytes(const ShapeInfo &G, int nv, int sp, int nmax, bool has_part
 {
     const size_t chunk = size_t(G.C) * nv * sp * G.P, part = size_t(2) * kSegEndRows * nv * G.P,
                  ex = size_t(4) * nv * sp * G.P, lu = size_t(5) * G.L * sp;
-    const size_t tile = size_t(nv * G.P + 1) * nmax;   // padded rows (SegShape::RS)
+    const size_t tile = size_t(nv) * G.P * size_t(nmax | 1);   // pencils n | 1 apart (SegShape::pencil_stride)
     if (G.XI) return sizeof(double) * (std::max(tile, lu + ex) + size_t(8) * G.SPMAX + (has_part ? part : 0));
     return sizeof(double) * (std::max(std::max(chunk, part), ex) + lu + part + size_t(8) * G.SPMAX);
 }

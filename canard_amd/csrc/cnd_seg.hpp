@@ -75,9 +75,11 @@ struct SegShape {
     static constexpr int NB = SCH == 0 ? 2 : 3;         // closure rows per end
     static constexpr int KR = L + 2 * H;                // rows streamed per lane
     static constexpr int NCH = (KR + C - 1) / C;        // staging chunks
-    // xi tiles in LDS: rows of NV * P values padded by one, so that lanes
-    // on consecutive rows of one pencil fall in different banks
-    static constexpr int RS = NV * P + 1;
+    // xi tiles in LDS: pencil after pencil, each n | 1 points long (an odd
+    // stride puts the lanes of neighbouring pencils in different banks),
+    // variable after variable; for odd n a point's place is its index in
+    // the tile
+    __host__ __device__ static constexpr int pencil_stride(int n) { return n | 1; }
     // LDS: a work area (chunk buffer [C][NV][SP][P], then partner points
     // [2][13][NV][P], then the exchange buffer [4][NV][SP][P]) and the
     // pencil-end values [2][13][NV][P].
@@ -90,14 +92,15 @@ struct SegShape {
     // then the segment transfer values [SPM][8] and the LU rows of the
     // launch's SP segments as their lanes hold them [SP][L][5]
     __host__ __device__ static constexpr int ends_doubles() { return 2 * kSegEndRows * NV * P; }
-    // (xi: the whole tile's sources [nmax][RS] replace the work area; once
+    // (xi: the whole tile's sources [NV][P][nmax | 1] replace the work area; once
     // the closure rows are formed, the LU rows and then the exchange buffer
     // [4][NV][SP][P] overlay the tile; the partner points [2][13][NV][P]
     // come last and only when has_part, some block having another block as
     // its partner along the sweep direction.)
     __host__ __device__ static constexpr size_t lds_bytes(int SP, int spm, int nmax, bool has_part)
     {
-        const size_t tile = size_t(RS) * nmax, ex4 = size_t(4) * NV * SP * P, lu = size_t(5) * L * SP;
+        const size_t tile = size_t(NV) * P * pencil_stride(nmax), ex4 = size_t(4) * NV * SP * P,
+                     lu = size_t(5) * L * SP;
         const size_t over = lu + ex4;
         if (XI)
             return sizeof(double) * ((tile > over ? tile : over) + size_t(8) * spm +
@@ -116,25 +119,9 @@ __device__ __forceinline__ void seg_slot(int slot, int SP, int &o, int &s, int &
 
 // xi sweeps (XI shapes run along xi only): the P pencils of a tile are
 // consecutive rows of the block, so point e of the tile (pencil e / n, row
-// e % n) sits at g0 + e, g0 the tile's first point.  A thread walks
-// e = t, t + nthr, ... and steps (pencil p, row r) by nthr instead of
-// dividing e by n.
-struct XiWalk {
-    int p, r, dq, dr, n;
-    __device__ XiWalk(int t, int nthr, int n_) : n(n_)
-    {
-        p = t / n;
-        r = t - p * n;
-        dq = nthr / n;
-        dr = nthr - dq * n;
-    }
-    __device__ void step()
-    {
-        r += dr;
-        p += dq;
-        if (r >= n) { r -= n; ++p; }
-    }
-};
+// e % n) sits at g0 + e, g0 the tile's first point; the LDS tile keeps the
+// same order (pencils n | 1 points apart), so staging and stores address
+// both by e.
 
 // One lane's view of its segment: rows a .. a + len - 1 of the line, held
 // right-aligned in Y[kb .. L - 1] (kb = L - len); Y[0 .. kb - 1] are
@@ -294,7 +281,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     const double *ST = tb.seg + 4 * int64_t(B.lu[SCH][d]);
     // LDS.  eta/zeta: work area (chunk buffer, then partner points, then the
     // exchange buffer), the pencil-end values, SM, LU rows.  xi: the whole
-    // tile's sources [n][RS] (once the closure rows are formed: the LU rows
+    // tile's sources [NV][P][n | 1] (once the closure rows are formed: the LU rows
     // and the exchange buffer; at the end: the results), SM, and the
     // partner points where some block has another block as partner along d.
     const int nlu = 5 * L * SP;
@@ -302,7 +289,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     double *ends = nullptr, *part = nullptr, *LUs = nullptr, *ex = nullptr;
     double *SM = nullptr;   // [SPM][8]
     if constexpr (XI) {
-        const int fsz = Sh::RS * n, exs4 = 4 * NV * SP * P;
+        const int fsz = NV * P * Sh::pencil_stride(n), exs4 = 4 * NV * SP * P;
         LUs = lds;
         ex = lds + nlu;
         SM = lds + (fsz > nlu + exs4 ? fsz : nlu + exs4);
@@ -315,8 +302,12 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         SM = ends + Sh::ends_doubles();
         LUs = SM + 8 * SPM;
     }
-    constexpr int RS = Sh::RS;                                              // xi: staged row stride
-    auto F = [&](int row) { return buf[row * RS + v * P + p]; };            // xi: staged source
+    // xi: staged source of the lane's variable and pencil at row; the place
+    // of tile point e (pencil e / n, row e % n) of variable 0; variable m
+    // follows FS doubles further on
+    const int NS = Sh::pencil_stride(n), FS = P * NS;
+    auto F = [&](int row) { return buf[(v * P + p) * NS + row]; };
+    auto place = [&](int e) { return (n & 1) ? e : e + e / n; };
     const bool pen_ok = tile.q0 + p < npen;
     const bool lane_ok = pen_ok && s < S;
     int a = 0, len = 0;
@@ -397,8 +388,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         constexpr int BX = (24 / RAWX) < 1 ? 1 : 24 / RAWX;
         const int tot = P * n;
         const int64_t g0 = pt(0, 0);
-        const int xlast = npen - 1 - tile.q0;   // last pencil of the tile inside the block
-        XiWalk xw(t, nthr, n);
+        const int elim = (npen - tile.q0 < P ? npen - tile.q0 : P) * n;   // points of the tile inside the block
         __syncthreads();
         for (int e0 = t; e0 < tot; e0 += nthr * BX) {
             typename Src::Raw raw[BX];
@@ -406,10 +396,9 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
 #pragma unroll
             for (int kk = 0; kk < BX; ++kk) {
                 const int e = e0 + kk * nthr;
-                const bool ok = e < tot && xw.p <= xlast;
+                const bool ok = e < elim;
                 src.load(ok ? g0 + e : g0, b, raw[kk]);
-                dst[kk] = ok ? xw.r * RS + xw.p : -1;
-                xw.step();
+                dst[kk] = ok ? place(e) : -1;
             }
 #pragma unroll
             for (int kk = 0; kk < BX; ++kk) {
@@ -417,7 +406,7 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
                 double f[NV];
                 src.finish(raw[kk], b, f);
 #pragma unroll
-                for (int m = 0; m < NV; ++m) buf[dst[kk] + m * P] = f[m];
+                for (int m = 0; m < NV; ++m) buf[dst[kk] + m * FS] = f[m];
             }
         }
         __syncthreads();
@@ -425,13 +414,10 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
         if constexpr (Src::kEmit) {
             // the staged values of the tile, written out point after point
             if (src.po) {
-                XiWalk xe(t, nthr, n);
-                for (int e = t; e < tot; e += nthr) {
-                    if (xe.p <= xlast) {
+                for (int e = t; e < elim; e += nthr) {
+                    const int li = place(e);
 #pragma unroll
-                        for (int m = 0; m < NV; ++m) src.po[m * src.NT + g0 + e] = buf[xe.r * RS + m * P + xe.p];
-                    }
-                    xe.step();
+                    for (int m = 0; m < NV; ++m) src.po[m * src.NT + g0 + e] = buf[li + m * FS];
                 }
             }
         }
@@ -848,14 +834,13 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
 #pragma unroll
             for (int k = 0; k < L; ++k)
                 if (have(k)) {
-                    buf[(a - kb + k) * RS + v * P + p] = Y[k];
+                    buf[(v * P + p) * NS + (a - kb + k)] = Y[k];
                 }
         }
         __syncthreads();
         const int tot = P * n;
         const int64_t g0 = pt(0, 0);
-        const int xlast = npen - 1 - tile.q0;
-        XiWalk xw(t, nthr, n);
+        const int elim = (npen - tile.q0 < P ? npen - tile.q0 : P) * n;   // points of the tile inside the block
         if constexpr (Sink::kReadBack) {
             // the sink adds the results to its sources: read those back for
             // kOutBatch points at a time, all loads in flight together
@@ -865,29 +850,26 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
 #pragma unroll
                 for (int kk = 0; kk < kOutBatch; ++kk) {
                     const int e = e0 + kk * nthr;
-                    const bool ok = e < tot && xw.p <= xlast;
-                    li[kk] = ok ? xw.r * RS + xw.p : -1;
+                    const bool ok = e < elim;
+                    li[kk] = ok ? place(e) : -1;
 #pragma unroll
                     for (int m = 0; m < NV; ++m) fs[kk][m] = sink.src(ok ? g0 + e : g0, m);
-                    xw.step();
                 }
 #pragma unroll
                 for (int kk = 0; kk < kOutBatch; ++kk) {
                     if (li[kk] < 0) continue;
                     const int64_t g = g0 + (e0 + kk * nthr);
 #pragma unroll
-                    for (int m = 0; m < NV; ++m) sink.put_raw(g, b, m, sink.fold(fs[kk][m], buf[li[kk] + m * P]));
+                    for (int m = 0; m < NV; ++m) sink.put_raw(g, b, m, sink.fold(fs[kk][m], buf[li[kk] + m * FS]));
                 }
             }
         } else {
-            for (int e = t; e < tot; e += nthr) {
-                if (xw.p <= xlast) {
-                    double x[NV];
+            for (int e = t; e < elim; e += nthr) {
+                const int li = place(e);
+                double x[NV];
 #pragma unroll
-                    for (int m = 0; m < NV; ++m) x[m] = buf[xw.r * RS + m * P + xw.p];
-                    sink.store(g0 + e, b, x);
-                }
-                xw.step();
+                for (int m = 0; m < NV; ++m) x[m] = buf[li + m * FS];
+                sink.store(g0 + e, b, x);
             }
         }
     }
