@@ -383,9 +383,9 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     const int nslots = C * SP * P;
     if constexpr (XI) {
         // stage all points of the tile at once, consecutive threads on
-        // consecutive points
-        constexpr int RAWX = int(sizeof(typename Src::Raw) / sizeof(double));
-        constexpr int BX = (24 / RAWX) < 1 ? 1 : 24 / RAWX;
+        // consecutive points: a thread has P n / (NV P SP) <= L / NV points
+        // (segments hold at most L), so BX slots take them in one round
+        constexpr int BX = (L + NV - 1) / NV;
         const int tot = P * n;
         const int64_t g0 = pt(0, 0);
         const int elim = (npen - tile.q0 < P ? npen - tile.q0 : P) * n;   // points of the tile inside the block
