@@ -26,10 +26,15 @@ def _shapes_for(d):
     return [-1] + [i for i, s in enumerate(shapes()) if s["xi"] == (d == 0)]
 
 
-def test_every_shape_sweeps_match_oracle():
-    blocks = [dict(lxi=100, let=60, lze=40, nbc=[[45, 45], [10, 10], [25, 20]], mcd=[[0, 0], [-1, -1], [-1, -1]])]
-    grids = [cases.gridtoy(100, 60, 40)]
-    npt = 101 * 61 * 41
+@pytest.mark.parametrize("nxyz", [(100, 60, 40), (99, 59, 41)])
+def test_every_shape_sweeps_match_oracle(nxyz):
+    """Every compiled shape against the oracle, on pencils of odd (101, 61,
+    41) and even (100, 60, 42) point counts: the xi tile in LDS keeps its
+    pencils n | 1 points apart."""
+    lx, ly, lz = nxyz
+    blocks = [dict(lxi=lx, let=ly, lze=lz, nbc=[[45, 45], [10, 10], [25, 20]], mcd=[[0, 0], [-1, -1], [-1, -1]])]
+    grids = [cases.gridtoy(lx, ly, lz)]
+    npt = (lx + 1) * (ly + 1) * (lz + 1)
     rng = np.random.default_rng(3)
     fields = [np.sin(grids[0][0]) * np.cos(2 * grids[0][1]) + 0.3 * rng.standard_normal(npt)]
     states = [cases.uniform_state(npt)]
@@ -50,11 +55,12 @@ def test_every_shape_sweeps_match_oracle():
     o.close()
 
 
-@pytest.mark.parametrize("pick", ["first", "last"])
-def test_every_shape_steps_match_oracle(pick):
+@pytest.mark.parametrize("pick,nxyz", [("first", (40, 36, 34)), ("last", (40, 36, 34)), ("first", (41, 35, 33)),
+                                       ("last", (41, 35, 33))])
+def test_every_shape_steps_match_oracle(pick, nxyz):
     """A few TGV steps with every kind pinned to the first (or last) compiled
-    shape that fits each direction."""
-    blocks, grids, states, params = tgv_case(0, nxyz=(40, 36, 34), pert=1e-3)
+    shape that fits each direction, on odd and on even point counts."""
+    blocks, grids, states, params = tgv_case(0, nxyz=nxyz, pert=1e-3)
     from canard_amd.solver import CanardError
     c = _case(blocks, grids, states, params)
     for d in range(3):
