@@ -424,7 +424,9 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
 #pragma unroll
         for (int kr = 0; kr < Sh::KR; ++kr) {
             const int row = a - kb - H + kr;
-            const double fv = (lane_ok && row >= 0 && row <= N) ? F(row) : 0.0;
+            // rows outside the line only reach the rows the closure rows
+            // replace (and lanes that store nothing): a clamped row serves
+            const double fv = F(row < 0 ? 0 : (row > N ? N : row));
 #pragma unroll
             for (int m = 0; m < 2 * H; ++m) w[m] = w[m + 1];
             w[2 * H] = fv;
