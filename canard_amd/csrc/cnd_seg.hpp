@@ -270,7 +270,9 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
     constexpr int H = Sh::H;
     const int nthr = NV * P * SP;
     const int t = threadIdx.x;
-    const int v = t / (P * SP), p = t % P, s = (t / P) % SP;
+    // (one variable: t < P SP, so v = 0 and s = t / P without dividing by
+    // the run-time SP)
+    const int v = NV == 1 ? 0 : t / (P * SP), p = t % P, s = NV == 1 ? t / P : (t / P) % SP;
     const int b = tile.b;
     const BlkDev &B = blks[b];
     const TabDev &T = tabs[B.inst];
