@@ -69,6 +69,7 @@ struct cnd_ctx {
     double tune_ms[CND_NKIND][3] = {};
     std::vector<Tile> hpt;
     Tile *dpt = nullptr;
+    int *dpt_inst = nullptr;   // the instance of each point tile
     // de: right-hand side as three 5-field buffers, one per flux direction
     // (summed in the update); fl: the 15 stored fluxes F[5 d + m] of viscous
     // stages, scratch for metrics, single sweeps and the autotuner otherwise

@@ -138,17 +138,18 @@ __global__ __launch_bounds__(kPts) void k_tstep(const BlkDev *__restrict__ blks,
     }
 }
 
-// per-instance maximum over the point tiles (one workgroup per instance)
-__global__ __launch_bounds__(kPts) void k_tstep_reduce(const BlkDev *__restrict__ blks, InstDev *__restrict__ inst,
-                                                      const Tile *__restrict__ tiles, int ntiles,
-                                                      const double *__restrict__ tmax)
+// per-instance maximum over the point tiles (one workgroup per instance;
+// tinst: the instance of each tile, so a tile costs independent loads only)
+__global__ __launch_bounds__(kPts) void k_tstep_reduce(InstDev *__restrict__ inst, const int *__restrict__ tinst,
+                                                      int ntiles, const double *__restrict__ tmax)
 {
     __shared__ double red[2][kPts / 64];
     InstDev &I = inst[blockIdx.x];
     if (I.n % 10 != 1) return;
     double a = 0.0, c = 0.0;
+#pragma unroll 4
     for (int k = threadIdx.x; k < ntiles; k += kPts)
-        if (blks[tiles[k].b].inst == int(blockIdx.x)) {
+        if (tinst[k] == int(blockIdx.x)) {
             a = fmax(a, tmax[2 * int64_t(k)]);
             c = fmax(c, tmax[2 * int64_t(k) + 1]);
         }
@@ -762,7 +763,7 @@ int enqueue_stage(cnd_ctx *c, int nk)
         Section sec(c, CND_SEC_TSTEP, 1);
         hipLaunchKernelGGL(k_tstep, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->met,
                            c->tmax, c->NT);
-        hipLaunchKernelGGL(k_tstep_reduce, dim3(unsigned(c->ninst)), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt,
+        hipLaunchKernelGGL(k_tstep_reduce, dim3(unsigned(c->ninst)), dim3(kPts), 0, c->st, c->dinst, c->dpt_inst,
                            int(c->hpt.size()), c->tmax);
         hipLaunchKernelGGL(k_sched, dim3(ib), dim3(64), 0, c->st, c->dinst, c->dtab, c->ninst);
     }
@@ -1037,7 +1038,9 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
                ok(dalloc(c, &c->cmf[d], 6 * c->FT[d])) && ok(dalloc(c, &c->surf[d], 10 * c->FT[d]));
     for (int d = 0; good && d < 3; ++d)
         for (int pi = 0; good && pi < kNumP; ++pi) good = ok(dalloc(c, &c->dstiles[pi][d], c->hstiles[pi][d].size()));
-    if (good) good = ok(dalloc(c, &c->dpt, c->hpt.size())) && ok(dalloc(c, &c->tmax, 2 * c->hpt.size()));
+    if (good)
+        good = ok(dalloc(c, &c->dpt, c->hpt.size())) && ok(dalloc(c, &c->dpt_inst, c->hpt.size())) &&
+               ok(dalloc(c, &c->tmax, 2 * c->hpt.size()));
     if (!good) { std::string m = c->err; cnd_destroy(c); return fail(nullptr, r ? r : CND_ENOMEM, m); }
     std::vector<TabDev> ht(ninst);
     for (int i = 0; i < ninst; ++i) {
@@ -1080,6 +1083,11 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         up(c->dlhs, lhs.data(), sizeof(double) * lhs.size());
     }
     up(c->dpt, c->hpt.data(), sizeof(Tile) * c->hpt.size());
+    {
+        std::vector<int> ti(c->hpt.size());
+        for (size_t k = 0; k < ti.size(); ++k) ti[k] = c->spec[c->hpt[k].b].inst;
+        up(c->dpt_inst, ti.data(), sizeof(int) * ti.size());
+    }
     if (!r) r = upload_inst(c);
     if (!r) {
         hipError_t e = hipMemset(c->de, 0, sizeof(double) * 15 * NT);
@@ -1120,7 +1128,7 @@ void cnd_destroy(cnd_ctx *c)
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->fl, c->met, c->ys,
-                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->gradf, c->dlu32, c->dlhs, c->jc.tgt, c->jc.src,
+                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->dpt_inst, c->gradf, c->dlu32, c->dlhs, c->jc.tgt, c->jc.src,
                     c->jc.first, c->jc.scale, c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp,
                     c->qb};
     for (void *p : ptrs) if (p) hipFree(p);
