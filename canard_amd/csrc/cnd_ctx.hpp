@@ -28,10 +28,11 @@ using namespace cnd;
 struct OutputState;   // output buffers and writer thread (cnd_output.hip)
 
 // ==== context ===================================================================
+// JunctionProgram on the device (one gather and one scatter launch)
 struct JuncDev {
     int n = 0;
     int64_t *tgt = nullptr, *src = nullptr;
-    int32_t *first = nullptr;
+    int32_t *first = nullptr, *tc = nullptr, *sc = nullptr, *write = nullptr;
     double *scale = nullptr, *tmp = nullptr;
 };
 
@@ -83,7 +84,7 @@ struct cnd_ctx {
     // tables and the FP64 banded LHS for the refinement
     float *gradf = nullptr, *dlu32 = nullptr, *dseg32[kNumL] = {};
     double *dlhs = nullptr;
-    JuncDev jc, je;
+    JuncDev jn;
     std::vector<char> grid_set;
     bool metrics_ready = false, state_in_qo = false;
     OutputState *out = nullptr;   // created by the first output call

@@ -224,19 +224,26 @@ __global__ void k_sched(InstDev *inst, const TabDev *tabs, int ninst)
 }
 
 // frame velocity for the update half of the stage (mo_canard_driver.f90:325-327)
-__global__ void k_stage_mid(InstDev *inst, int ninst, int nk)
+__device__ void stage_mid(InstDev &I, int nk)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ninst) return;
-    InstDev &I = inst[i];
     const int c = nk - 1 < 1 ? nk - 1 : 1;
     movef(I, (I.dt * double(c)) / double(4 - nk + 2), I.dt / double(4 - nk + 1));
 }
 
-__global__ void k_end_step(InstDev *inst, int ninst)
+__global__ void k_stage_mid(InstDev *inst, int ninst, int nk)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
+    stage_mid(inst[i], nk);
+}
+
+// End of step; mid4: first the frame velocity of stage 4's update half,
+// where no kernel of stage 4 reads it.
+__global__ void k_end_step(InstDev *inst, int ninst, int mid4)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    if (mid4) stage_mid(inst[i], 4);
     inst[i].n = inst[i].n + 1;
     inst[i].timo = inst[i].timo + inst[i].dt;
 }
@@ -495,48 +502,85 @@ __global__ __launch_bounds__(kLine) void k_wall(const BlkDev *__restrict__ blks,
 }
 
 // ---- averaging of duplicated points ------------------------------------------
-// junction (corner / edge) averaging: gather then scatter
-__global__ void k_junc_gather(const int64_t *__restrict__ tgt, const int32_t *__restrict__ first,
-                              const int64_t *__restrict__ src, const double *__restrict__ scale,
-                              const double *__restrict__ q, double *__restrict__ tmp, int n, int64_t NT)
+// Junction averaging (average_point, then average_line: mo_domdcomp.f90:439-559)
+// as one gather and one scatter over a JunctionProgram: an edge entry reads a
+// point that corner averaging changes through that corner's entry, which is
+// recomputed in the same order of additions.
+__device__ __forceinline__ double junc_corner(const JuncDev &J, const double *q, int j, int m, int64_t NT)
+{
+    double acc = q[m * NT + J.tgt[j]];
+    for (int k = J.first[j]; k < J.first[j + 1]; ++k) acc = acc + q[m * NT + J.src[k]];
+    return J.scale[j] * acc;
+}
+
+__global__ void k_junc_gather(const JuncDev J, const double *__restrict__ q, int64_t NT)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t g = tgt[i];
+    if (i >= J.n || !J.write[i]) return;
     for (int m = 0; m < 5; ++m) {
-        double acc = q[m * NT + g];
-        for (int k = first[i]; k < first[i + 1]; ++k) acc = acc + q[m * NT + src[k]];
-        tmp[m * int64_t(n) + i] = scale[i] * acc;
+        double acc = J.tc[i] < 0 ? q[m * NT + J.tgt[i]] : junc_corner(J, q, J.tc[i], m, NT);
+        for (int k = J.first[i]; k < J.first[i + 1]; ++k)
+            acc = acc + (J.sc[k] < 0 ? q[m * NT + J.src[k]] : junc_corner(J, q, J.sc[k], m, NT));
+        J.tmp[m * int64_t(J.n) + i] = J.scale[i] * acc;
     }
 }
 
-__global__ void k_junc_scatter(const int64_t *__restrict__ tgt, const double *__restrict__ tmp, double *q, int n,
-                               int64_t NT)
+__global__ void k_junc_scatter(const JuncDev J, double *q, int64_t NT)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (int m = 0; m < 5; ++m) q[m * NT + tgt[i]] = tmp[m * int64_t(n) + i];
+    if (i >= J.n || !J.write[i]) return;
+    for (int m = 0; m < 5; ++m) q[m * NT + J.tgt[i]] = J.tmp[m * int64_t(J.n) + i];
 }
 
-// average_surface (mo_gcbc.f90:342-422): face snapshot, then running means
-// face by face in the reference's (direction, side) order.
-__global__ __launch_bounds__(kLine) void k_surf_snap(const BlkDev *__restrict__ blks, const Tile *__restrict__ tiles,
-                                                    int d, const double *q, double *sv, int64_t NT, int64_t FT)
+// The face tiles of all three directions in one launch: blockIdx.x runs over
+// the tiles of direction 0, then 1, then 2.
+struct FaceSet {
+    const Tile *tiles[3];
+    int n0, n01;
+    double *sv[3];
+    int64_t FT[3];
+    __device__ Tile tile(int bx, int &d) const
+    {
+        d = bx < n0 ? 0 : (bx < n01 ? 1 : 2);
+        return tiles[d][bx - (d == 0 ? 0 : (d == 1 ? n0 : n01))];
+    }
+};
+
+// pencil of direction d through point (i, j, k) (the inverse of pencil_of)
+__device__ __forceinline__ int pencil_at(const BlkDev &B, int d, const int idx[3])
 {
-    const Tile t = tiles[blockIdx.x];
+    return d == 0 ? idx[1] + B.n[1] * idx[2] : (d == 1 ? idx[0] + B.n[0] * idx[2] : idx[0] + B.n[0] * idx[1]);
+}
+
+__device__ __forceinline__ bool on_avg_face(const BlkDev &B, int d, int s, const int idx[3])
+{
+    return is_avg(B.bc[d][s]) && idx[d] == (s == 0 ? 0 : B.n[d] - 1);
+}
+
+// average_surface (mo_gcbc.f90:342-422): snapshot of every averaged face, then
+// running means face by face in the reference's (direction, side) order.
+__global__ __launch_bounds__(kLine) void k_surf_snap(const BlkDev *__restrict__ blks, const FaceSet fs,
+                                                    const double *q, int64_t NT)
+{
+    int d;
+    const Tile t = fs.tile(blockIdx.x, d);
     const BlkDev &B = blks[t.b];
     const int side = blockIdx.y, qq = t.q0 + threadIdx.x;
     if (qq >= npencils(B, d) || !is_avg(B.bc[d][side])) return;
     int64_t g, st;
     face_point(B, d, qq, side, g, st);
     const int64_t fi = B.foff[d] + qq;
-    for (int m = 0; m < 5; ++m) sv[(side * 5 + m) * FT + fi] = q[m * NT + g];
+    for (int m = 0; m < 5; ++m) fs.sv[d][(side * 5 + m) * fs.FT[d] + fi] = q[m * NT + g];
 }
 
-__global__ __launch_bounds__(kLine) void k_surf_avg(const BlkDev *__restrict__ blks, const Tile *__restrict__ tiles,
-                                                   int d, double *q, const double *sv, int64_t NT, int64_t FT)
+// One thread per (face, point); a point on several averaged faces of its
+// block (an edge or corner) is written once, by the thread of its last face,
+// which applies the running means of all of them in order.
+__global__ __launch_bounds__(kLine) void k_surf_avg(const BlkDev *__restrict__ blks, const FaceSet fs, double *q,
+                                                   int64_t NT)
 {
-    const Tile t = tiles[blockIdx.x];
+    int d;
+    const Tile t = fs.tile(blockIdx.x, d);
     const BlkDev &B = blks[t.b];
     const int side = blockIdx.y, qq = t.q0 + threadIdx.x;
     if (qq >= npencils(B, d) || !is_avg(B.bc[d][side])) return;
@@ -547,20 +591,21 @@ __global__ __launch_bounds__(kLine) void k_surf_avg(const BlkDev *__restrict__ b
     idx[0] = int(lloc % B.n[0]);
     idx[1] = int((lloc / B.n[0]) % B.n[1]);
     idx[2] = int(lloc / (int64_t(B.n[0]) * B.n[1]));
-    // running count: faces of this block, up to and including (d, side), that
-    // hold the point
+    for (int dd = d + 1; dd < 3; ++dd)
+        if (on_avg_face(B, dd, 0, idx) || on_avg_face(B, dd, 1, idx)) return;
+    double v[5];
+    for (int m = 0; m < 5; ++m) v[m] = q[m * NT + g];
     double cnt = 1.0;
     for (int f = 0; f <= 2 * d + side; ++f) {
         const int dd = f / 2, s = f % 2;
-        if (is_avg(B.bc[dd][s]) && idx[dd] == (s == 0 ? 0 : B.n[dd] - 1)) cnt = cnt + 1.0;
+        if (!on_avg_face(B, dd, s, idx)) continue;
+        cnt = cnt + 1.0;
+        const double r2 = 1.0 / cnt;
+        const int64_t pi = blks[B.part[dd][s]].foff[dd] + pencil_at(B, dd, idx);
+        const double *sv = fs.sv[dd] + (1 - s) * 5 * fs.FT[dd] + pi;
+        for (int m = 0; m < 5; ++m) v[m] = r2 * ((cnt - 1.0) * v[m] + sv[m * fs.FT[dd]]) + 0.0 * v[m];
     }
-    const double r2 = 1.0 / cnt;
-    const BlkDev &P = blks[B.part[d][side]];
-    const int64_t pi = P.foff[d] + qq;
-    for (int m = 0; m < 5; ++m) {
-        const double v = q[m * NT + g];
-        q[m * NT + g] = r2 * ((cnt - 1.0) * v + sv[((1 - side) * 5 + m) * FT + pi]) + 0.0 * v;
-    }
+    for (int m = 0; m < 5; ++m) q[m * NT + g] = v[m];
 }
 
 // ---- grid metrics (mo_grid.f90:107-186) ----------------------------------------
@@ -807,27 +852,30 @@ int enqueue_stage(cnd_ctx *c, int nk)
         hipLaunchKernelGGL(k_update, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->qo, c->de,
                            c->met, c->sponge, c->qa, c->NT, nk);
     }
-    hipLaunchKernelGGL(k_stage_mid, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, nk);
+    // The update half's frame velocity is read only by the wall condition
+    // (and, after the step, by the output); without walls the next stage's
+    // frame velocity replaces it unread, and stage 4's is set at the end of
+    // the step.
+    if (c->any_wall) hipLaunchKernelGGL(k_stage_mid, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, nk);
     Section avg(c, CND_SEC_AVG, 1);
     if (c->any_wall)
         for (int d = 0; d < 3; ++d)
             for (int s = 0; s < 2; ++s)
                 hipLaunchKernelGGL(k_wall, tiles_grid(c, d, 1), dim3(kLine), 0, c->st, c->dblk, c->dinst,
                                    c->dtiles[d], d, s, c->qa, c->NT);
-    for (JuncDev *j : {&c->jc, &c->je}) {
-        if (j->n == 0) continue;
-        const unsigned nb = unsigned((j->n + 127) / 128);
-        hipLaunchKernelGGL(k_junc_gather, dim3(nb), dim3(128), 0, c->st, j->tgt, j->first, j->src, j->scale, c->qa,
-                           j->tmp, j->n, c->NT);
-        hipLaunchKernelGGL(k_junc_scatter, dim3(nb), dim3(128), 0, c->st, j->tgt, j->tmp, c->qa, j->n, c->NT);
+    if (c->jn.n > 0) {
+        const unsigned nb = unsigned((c->jn.n + 127) / 128);
+        hipLaunchKernelGGL(k_junc_gather, dim3(nb), dim3(128), 0, c->st, c->jn, c->qa, c->NT);
+        hipLaunchKernelGGL(k_junc_scatter, dim3(nb), dim3(128), 0, c->st, c->jn, c->qa, c->NT);
     }
     if (c->any_avg) {
-        for (int d = 0; d < 3; ++d)
-            hipLaunchKernelGGL(k_surf_snap, tiles_grid(c, d, 2), dim3(kLine), 0, c->st, c->dblk, c->dtiles[d], d,
-                               c->qa, c->surf[d], c->NT, c->FT[d]);
-        for (int d = 0; d < 3; ++d)
-            hipLaunchKernelGGL(k_surf_avg, tiles_grid(c, d, 2), dim3(kLine), 0, c->st, c->dblk, c->dtiles[d], d,
-                               c->qa, c->surf[d], c->NT, c->FT[d]);
+        FaceSet fs;
+        for (int d = 0; d < 3; ++d) { fs.tiles[d] = c->dtiles[d]; fs.sv[d] = c->surf[d]; fs.FT[d] = c->FT[d]; }
+        fs.n0 = ntiles(c, 0);
+        fs.n01 = ntiles(c, 0) + ntiles(c, 1);
+        const dim3 grid(unsigned(fs.n01 + ntiles(c, 2)), 2);
+        hipLaunchKernelGGL(k_surf_snap, grid, dim3(kLine), 0, c->st, c->dblk, fs, c->qa, c->NT);
+        hipLaunchKernelGGL(k_surf_avg, grid, dim3(kLine), 0, c->st, c->dblk, fs, c->qa, c->NT);
     }
     return check_launch(c);
 }
@@ -835,7 +883,7 @@ int enqueue_stage(cnd_ctx *c, int nk)
 int enqueue_end(cnd_ctx *c)
 {
     const unsigned ib = unsigned((c->ninst + 63) / 64);
-    hipLaunchKernelGGL(k_end_step, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst);
+    hipLaunchKernelGGL(k_end_step, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, c->any_wall ? 0 : 1);
     if (c->recording) {
         hipLaunchKernelGGL(k_record, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, c->qo, c->qa,
                            c->qb, c->NT);
@@ -1094,22 +1142,24 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         if (e == hipSuccess) e = hipMemset(c->fl, 0, sizeof(double) * 15 * NT);
         if (e != hipSuccess) r = fail(c, CND_EDEVICE, "memset");
     }
-    // junction plans (search_point / search_line)
+    // junction averaging (search_point / search_line, average_point / average_line)
     if (!r) {
         JunctionPlan pc, pe;
         junction_plans(c->spec, c->off, pc, pe);
-        for (int k = 0; k < 2 && !r; ++k) {
-            JunctionPlan &p = k == 0 ? pc : pe;
-            JuncDev &j = k == 0 ? c->jc : c->je;
-            j.n = int(p.target.size());
-            if (j.n == 0) continue;
-            if ((r = dalloc(c, &j.tgt, p.target.size())) || (r = dalloc(c, &j.src, p.src.size())) ||
-                (r = dalloc(c, &j.first, p.first.size())) || (r = dalloc(c, &j.scale, p.scale.size())) ||
-                (r = dalloc(c, &j.tmp, 5 * size_t(j.n))))
-                break;
+        const JunctionProgram p = junction_program(pc, pe);
+        JuncDev &j = c->jn;
+        j.n = int(p.target.size());
+        if (j.n > 0 &&
+            !((r = dalloc(c, &j.tgt, p.target.size())) || (r = dalloc(c, &j.src, p.src.size())) ||
+              (r = dalloc(c, &j.first, p.first.size())) || (r = dalloc(c, &j.tc, p.tc.size())) ||
+              (r = dalloc(c, &j.sc, p.sc.size())) || (r = dalloc(c, &j.write, p.write.size())) ||
+              (r = dalloc(c, &j.scale, p.scale.size())) || (r = dalloc(c, &j.tmp, 5 * size_t(j.n))))) {
             up(j.tgt, p.target.data(), sizeof(int64_t) * p.target.size());
-            up(j.src, p.src.data(), sizeof(int64_t) * std::max<size_t>(p.src.size(), 0));
+            up(j.src, p.src.data(), sizeof(int64_t) * p.src.size());
             up(j.first, p.first.data(), sizeof(int32_t) * p.first.size());
+            up(j.tc, p.tc.data(), sizeof(int32_t) * p.tc.size());
+            up(j.sc, p.sc.data(), sizeof(int32_t) * p.sc.size());
+            up(j.write, p.write.data(), sizeof(int32_t) * p.write.size());
             up(j.scale, p.scale.data(), sizeof(double) * p.scale.size());
         }
     }
@@ -1128,9 +1178,8 @@ void cnd_destroy(cnd_ctx *c)
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->fl, c->met, c->ys,
-                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->dpt_inst, c->gradf, c->dlu32, c->dlhs, c->jc.tgt, c->jc.src,
-                    c->jc.first, c->jc.scale, c->jc.tmp, c->je.tgt, c->je.src, c->je.first, c->je.scale, c->je.tmp,
-                    c->qb};
+                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->dpt_inst, c->gradf, c->dlu32, c->dlhs, c->jn.tgt, c->jn.src,
+                    c->jn.first, c->jn.tc, c->jn.sc, c->jn.write, c->jn.scale, c->jn.tmp, c->qb};
     for (void *p : ptrs) if (p) hipFree(p);
     for (int li = 0; li < kNumL; ++li) {
         if (c->dseg[li]) hipFree(c->dseg[li]);

@@ -480,4 +480,44 @@ void junction_plans(const std::vector<BlockSpec> &blocks, const std::vector<int6
     }
 }
 
+// average_line packs every edge before it averages any (mo_domdcomp.f90:505-529)
+// and then writes edge by edge (:533-557), so a block corner on several
+// averaged edges ends with the last edge's value; average_point runs before
+// it (mo_canard_driver.f90:347-351).
+JunctionProgram junction_program(const JunctionPlan &corners, const JunctionPlan &edges)
+{
+    JunctionProgram p;
+    const int nc = int(corners.target.size()), ne = int(edges.target.size());
+    std::map<int64_t, int> corner_of, last_edge;
+    for (int i = 0; i < nc; ++i) corner_of[corners.target[i]] = i;
+    for (int i = 0; i < ne; ++i) last_edge[edges.target[i]] = i;
+    auto cidx = [&](int64_t g) { auto it = corner_of.find(g); return it == corner_of.end() ? -1 : it->second; };
+    p.ncorner = nc;
+    p.first.push_back(0);
+    for (int i = 0; i < nc; ++i) {
+        p.target.push_back(corners.target[i]);
+        for (int k = corners.first[i]; k < corners.first[i + 1]; ++k) {
+            p.src.push_back(corners.src[k]);
+            p.sc.push_back(-1);
+        }
+        p.first.push_back(int32_t(p.src.size()));
+        p.scale.push_back(corners.scale[i]);
+        p.tc.push_back(-1);
+        p.write.push_back(last_edge.count(corners.target[i]) ? 0 : 1);
+    }
+    for (int i = 0; i < ne; ++i) {
+        if (last_edge[edges.target[i]] != i) continue;
+        p.target.push_back(edges.target[i]);
+        for (int k = edges.first[i]; k < edges.first[i + 1]; ++k) {
+            p.src.push_back(edges.src[k]);
+            p.sc.push_back(cidx(edges.src[k]));
+        }
+        p.first.push_back(int32_t(p.src.size()));
+        p.scale.push_back(edges.scale[i]);
+        p.tc.push_back(cidx(edges.target[i]));
+        p.write.push_back(1);
+    }
+    return p;
+}
+
 }  // namespace cnd

@@ -107,4 +107,19 @@ struct JunctionPlan {
 void junction_plans(const std::vector<BlockSpec> &blocks, const std::vector<int64_t> &offsets,
                     JunctionPlan &corners, JunctionPlan &edges);
 
+// Corner and edge averaging as one gather and one scatter.  Entries
+// [0, ncorner) are the corner plan's, the rest the edge plan's.  An edge
+// entry reads a corner target through that corner's entry (tc for its
+// target, sc per source; -1: the point itself), i.e. the value corner
+// averaging leaves there.  Of several edge entries for one point only the
+// last (the one the reference writes last) is kept; a corner entry whose
+// point an edge entry also writes has write = 0.
+struct JunctionProgram {
+    int ncorner = 0;
+    std::vector<int64_t> target, src;
+    std::vector<int32_t> first, tc, sc, write;
+    std::vector<double> scale;
+};
+JunctionProgram junction_program(const JunctionPlan &corners, const JunctionPlan &edges);
+
 }  // namespace cnd
