@@ -119,6 +119,46 @@ def test_tgv_steps_match_oracle(n, steps, pert):
     o.close()
 
 
+def test_eight_block_periodic_tgv_matches_oracle():
+    """A 2 x 2 x 2 arrangement of 16^3-cell blocks tiling the periodic TGV box
+    (z falls with zeta as in gridtoy), every interface joining two different
+    blocks (BC 35 inside the box, BC 45 across its periodic wrap): corners are
+    shared by eight blocks and edges by four, so the corner and edge
+    averaging of every stage acts on every block edge (and corners lie on
+    several averaged edges of their block).  8 steps against the oracle,
+    1e-12 per variable; a second context gives bit-identical states."""
+    n, h = 16, np.pi / 16
+    idx = lambda i, j, k: i + 2 * j + 4 * k
+    blocks, grids, states = [], [], []
+    kk, jj, ii = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    for k in range(2):
+        for j in range(2):
+            for i in range(2):
+                # faces across the box's periodic wrap are BC 45 (coordinate jump)
+                nbc = [[45 if c == 0 else 35, 45 if c == 1 else 35] for c in (i, j, k)]
+                blocks.append(dict(lxi=n, let=n, lze=n, nbc=nbc,
+                                   mcd=[[idx(1 - i, j, k)] * 2, [idx(i, 1 - j, k)] * 2, [idx(i, j, 1 - k)] * 2]))
+                xyz = np.stack([(ii.ravel() + i * n) * h, (jj.ravel() + j * n) * h, -(kk.ravel() + k * n) * h])
+                grids.append(np.ascontiguousarray(xyz, dtype=np.float64))
+                states.append(cases.tgv_state(grids[-1], (n + 1,) * 3, mach=0.1))
+    params = dict(fltk=0.49, fltrbc=0.9, reoo=1600.0 / 0.1, tempoo=300.0, cfl=0.8, viscous=1)
+    print("[steps] eight-block periodic tgv", flush=True)
+    c = _case(blocks, grids, states, params)
+    c2 = _case(blocks, grids, states, params)
+    o = oracle_for(blocks, grids, states, params)
+    c.step(8)
+    c2.step(8)
+    o.run(8)
+    for b in range(8):
+        err = rel_err(c.state(b), o.state(b))
+        print("  block", b, "rel err per variable", err)
+        assert np.all(err < 1e-12), (b, err)
+        assert np.array_equal(c.state(b), c2.state(b)), b
+    c.close()
+    c2.close()
+    o.close()
+
+
 def test_noncubic_periodic_matches_oracle():
     blocks, grids, states, params = tgv_case(0, nxyz=(20, 16, 24), pert=1e-3)
     print("[steps] noncubic 20x16x24", flush=True)

@@ -334,6 +334,57 @@ def test_surface_averaging_is_the_mean_of_both_sides():
     o.close()
 
 
+def test_junctions_of_eight_blocks_average_every_copy():
+    """average_point and average_line (mo_domdcomp.f90:439-559), then
+    average_surface, on a periodic 2 x 2 x 2 arrangement of blocks, each of
+    whose interfaces joins two different blocks: every physical point ends
+    with the mean of all its copies before averaging -- 8 copies of a
+    corner, 4 of an edge point, 2 of a face point (rtol 1e-14).  The
+    junction partners come from the merged per-face partner lists and the
+    matching of messages by tag and posting order; a corner or edge matched
+    to the wrong block or item, a wrong count, or a point averaged twice
+    gives another value."""
+    n = 12
+    idx = lambda i, j, k: i + 2 * j + 4 * k
+    blocks, grids, states = [], [], []
+    rng = np.random.default_rng(21)
+    for k in range(2):
+        for j in range(2):
+            for i in range(2):
+                blocks.append(dict(lxi=n, let=n, lze=n, nbc=[[35, 35]] * 3,
+                                   mcd=[[idx(1 - i, j, k)] * 2, [idx(i, 1 - j, k)] * 2, [idx(i, j, 1 - k)] * 2]))
+                grids.append(cases.gridtoy(n, n, n))
+                states.append(1.0 + 0.1 * rng.standard_normal((5, (n + 1) ** 3)))
+    o = oracle_for(blocks, grids, states, dict(reoo=1000.0))
+    o.stage_tail()
+    post = [o.state(b).reshape(5, n + 1, n + 1, n + 1) for b in range(8)]
+    pre = [s.reshape(5, n + 1, n + 1, n + 1) for s in states]
+    # copies of global point (I, J, K), 0 <= I, J, K < 2n: block (i, j, k)
+    # holds local index l = I - i n (mod 2n) along each direction, 0 <= l <= n
+    copies = {}
+    for k in range(2):
+        for j in range(2):
+            for i in range(2):
+                b = idx(i, j, k)
+                for lk in range(n + 1):
+                    for lj in range(n + 1):
+                        for li in range(n + 1):
+                            key = ((li + i * n) % (2 * n), (lj + j * n) % (2 * n), (lk + k * n) % (2 * n))
+                            copies.setdefault(key, []).append((b, lk, lj, li))
+    sizes = {}
+    for key, cp in copies.items():
+        sizes[len(cp)] = sizes.get(len(cp), 0) + 1
+        if len(cp) == 1:
+            b, lk, lj, li = cp[0]
+            assert np.array_equal(post[b][:, lk, lj, li], pre[b][:, lk, lj, li])
+            continue
+        mean = np.mean([pre[b][:, lk, lj, li] for b, lk, lj, li in cp], axis=0)
+        for b, lk, lj, li in cp:
+            np.testing.assert_allclose(post[b][:, lk, lj, li], mean, rtol=1e-14, atol=0, err_msg=str((key, b)))
+    assert sorted(sizes) == [1, 2, 4, 8] and sizes[8] == 8
+    o.close()
+
+
 def test_wall_conditions_closed_form():
     """wall_condition_update (mo_gcbc.f90:424-459): on an inviscid wall (BC
     20) density and momentum stay and the energy is reset so that p = rho^gam
