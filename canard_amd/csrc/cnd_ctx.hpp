@@ -47,6 +47,11 @@ struct cnd_ctx {
     long long cells = 0;
     int64_t npts = 0;   // stored points (NT, the field stride, also counts the alignment padding)
     hipStream_t st = nullptr;
+    // side stream for sweeps that do not depend on each other (eta with
+    // zeta gradients, xi with eta fluxes); ev_fork/ev_join order it with st
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool overlap = true;   // CANARD_OVERLAP=0: every sweep on st
     BlkDev *dblk = nullptr;
     TabDev *dtab = nullptr;
     InstDev *dinst = nullptr;

@@ -799,6 +799,28 @@ int enqueue_filter(cnd_ctx *c)
     return CND_OK;
 }
 
+// Sweeps that neither read what the other writes nor share scratch (the
+// tiled sweeps' ys) run side by side: launch() goes to the side stream
+// after everything queued so far on st; join_side makes st wait for it.
+template <class F>
+int on_side(cnd_ctx *c, F launch)
+{
+    HIPX(c, hipEventRecord(c->ev_fork, c->st));
+    HIPX(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+    std::swap(c->st, c->st2);
+    const int r = launch();
+    std::swap(c->st, c->st2);
+    if (r) return r;
+    HIPX(c, hipEventRecord(c->ev_join, c->st2));
+    return CND_OK;
+}
+
+int join_side(cnd_ctx *c)
+{
+    HIPX(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
+    return CND_OK;
+}
+
 int enqueue_stage(cnd_ctx *c, int nk)
 {
     const double *cur = nk == 1 ? c->qo : c->qa;
@@ -814,9 +836,17 @@ int enqueue_stage(cnd_ctx *c, int nk)
     }
     int r;
     if (c->any_visc) {
+        // the eta sweep beside the zeta sweep when it is a segmented one
         Section sec(c, CND_SEC_GRAD, 3);
-        for (int d = 0; d < 3; ++d)
-            if ((r = run_grad(c, d, cur))) return r;
+        if ((r = run_grad(c, 0, cur))) return r;
+        if (c->overlap && (c->precision == CND_MIXED || seg_on(c, CND_KIND_GRAD, 1))) {
+            if ((r = on_side(c, [&] { return run_grad(c, 1, cur); })) || (r = run_grad(c, 2, cur)) ||
+                (r = join_side(c)))
+                return r;
+        } else {
+            for (int d = 1; d < 3; ++d)
+                if ((r = run_grad(c, d, cur))) return r;
+        }
     }
     if (c->any_visc) {
         Section sec(c, CND_SEC_STRESS, 1);
@@ -829,9 +859,17 @@ int enqueue_stage(cnd_ctx *c, int nk)
     }
     const bool fused = fuse_possible(c) && (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, 2));
     {
+    // the eta flux sweep beside the xi (and an unfused zeta) sweep when it is
+    // a segmented one
     Section sec(c, CND_SEC_FLUX, fused ? 2 : 3);
-    for (int d = 0; d < (fused ? 2 : 3); ++d)
-        if ((r = run_flux(c, d, cur))) return r;
+    if (c->overlap && (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, 1))) {
+        if ((r = on_side(c, [&] { return run_flux(c, 1, cur); })) || (r = run_flux(c, 0, cur)) ||
+            (!fused && (r = run_flux(c, 2, cur))) || (r = join_side(c)))
+            return r;
+    } else {
+        for (int d = 0; d < (fused ? 2 : 3); ++d)
+            if ((r = run_flux(c, d, cur))) return r;
+    }
     }
     if (c->any_gcbc) {
         Section sec(c, CND_SEC_GCBC, 9);
@@ -1030,6 +1068,8 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         c->tune_on = !c->tiled_only && !(tu && std::string(tu) == "0");
         const char *fu = std::getenv("CANARD_FUSE_UPDATE");
         c->fuse_update = !(fu && std::string(fu) == "0");
+        const char *ov = std::getenv("CANARD_OVERLAP");
+        c->overlap = !(ov && std::string(ov) == "0");
         {
             int dev = 0, lds = 0;
             if (hipGetDevice(&dev) == hipSuccess &&
@@ -1067,6 +1107,12 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(nullptr, CND_EDEVICE, "stream creation failed");
+    }
+    if (hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        cnd_destroy(c);
+        return fail(nullptr, CND_EDEVICE, "side stream creation failed");
     }
     const int64_t NT = c->NT;
     bool good = ok(dalloc(c, &c->dblk, nblocks)) && ok(dalloc(c, &c->dtab, ninst)) && ok(dalloc(c, &c->dinst, ninst)) &&
@@ -1173,6 +1219,7 @@ void cnd_destroy(cnd_ctx *c)
 {
     if (!c) return;
     if (c->st) hipStreamSynchronize(c->st);
+    if (c->st2) hipStreamSynchronize(c->st2);
     output_release(c);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
@@ -1193,6 +1240,9 @@ void cnd_destroy(cnd_ctx *c)
         if (c->cmf[d]) hipFree(c->cmf[d]);
         if (c->surf[d]) hipFree(c->surf[d]);
     }
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->st2) hipStreamDestroy(c->st2);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
 }
