@@ -69,6 +69,7 @@ struct cnd_ctx {
     int seg_sp[CND_NKIND][3] = {};
     bool tuned = false, tune_on = true, tiled_only = false;
     bool fuse_update = true;   // last viscous flux sweep writes the RK update (CANARD_FUSE_UPDATE=0: separate)
+    int fuse_dir = 0;          // the direction of that sweep, 0 (xi) or 2 (zeta; CANARD_FUSE_DIR=2)
     bool recording = false;    // cnd_record: time averages accumulated in qb
     double *qb = nullptr;
     int lds_max = 65536;   // dynamic LDS per workgroup the device allows
@@ -215,8 +216,9 @@ inline int kind_nv(const cnd_ctx *c, int kind, int d)
 
 
 // Viscous stages without characteristic faces or sponge fold the update
-// into the zeta flux sweep when that sweep is segmented (GCBC needs all
-// three flux derivatives before the update; the sponge stays in k_update).
+// into the flux sweep of direction fuse_dir, run after the other two, when
+// that sweep is segmented (GCBC needs all three flux derivatives before the
+// update; the sponge stays in k_update).
 inline bool fuse_possible(const cnd_ctx *c)
 {
     return c->fuse_update && c->any_visc && !c->any_gcbc && !c->any_sponge;

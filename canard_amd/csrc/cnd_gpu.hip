@@ -857,18 +857,19 @@ int enqueue_stage(cnd_ctx *c, int nk)
             hipLaunchKernelGGL(k_visc_flux<double>, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt,
                                cur, c->grad, c->met, c->fl, c->NT);
     }
-    const bool fused = fuse_possible(c) && (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, 2));
+    const bool fused = fuse_possible(c) && (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, c->fuse_dir));
     {
-    // the eta flux sweep beside the xi (and an unfused zeta) sweep when it is
-    // a segmented one
+    // the flux sweeps the update does not take in, the eta one beside the
+    // others when it is a segmented one
     Section sec(c, CND_SEC_FLUX, fused ? 2 : 3);
+    const int other = fused && c->fuse_dir == 0 ? 2 : 0;
     if (c->overlap && (c->precision == CND_MIXED || seg_on(c, CND_KIND_FLUX, 1))) {
-        if ((r = on_side(c, [&] { return run_flux(c, 1, cur); })) || (r = run_flux(c, 0, cur)) ||
+        if ((r = on_side(c, [&] { return run_flux(c, 1, cur); })) || (r = run_flux(c, other, cur)) ||
             (!fused && (r = run_flux(c, 2, cur))) || (r = join_side(c)))
             return r;
     } else {
-        for (int d = 0; d < (fused ? 2 : 3); ++d)
-            if ((r = run_flux(c, d, cur))) return r;
+        for (int d = 0; d < 3; ++d)
+            if ((!fused || d != c->fuse_dir) && (r = run_flux(c, d, cur))) return r;
     }
     }
     if (c->any_gcbc) {
@@ -1094,6 +1095,11 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
                                 "points, and CANARD_SWEEP not set to tiled)");
                 }
             }
+        // the update goes with the xi flux sweep, whose tile is one contiguous
+        // run of points (its operands stream like a pointwise kernel's), unless
+        // only the zeta one is segmented or CANARD_FUSE_DIR=2
+        const char *fd = std::getenv("CANARD_FUSE_DIR");
+        c->fuse_dir = (fd && std::string(fd) == "2") || !seg_on(c, CND_KIND_FLUX, 0) ? 2 : 0;
     }
     // forward values of up to 9 variables per pencil (metric filtering)
     for (int d = 0; d < 3; ++d)
@@ -1375,20 +1381,20 @@ static int tune(cnd_ctx *c)
     HIPX(c, hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(c, CND_EDEVICE, "tune: event"); }
     int r = CND_OK;
-    // the zeta flux sweep is timed as the step runs it: fused with the
-    // update (into the gradient scratch) where fusion applies
+    // the flux sweep that carries the update is timed as the step runs it:
+    // fused with the update (into the gradient scratch) where fusion applies
     const bool fz = fuse_possible(c);
     auto run = [&](int kind, int d) {
         if (kind == CND_KIND_FILTER) return run_filter_field(c, d, c->qa, c->fl, 5, mix);
         if (kind == CND_KIND_GRAD) return run_grad(c, d, c->qa);
-        if (d == 2 && fz) return run_flux_update(c, 1, c->grad);
+        if (d == c->fuse_dir && fz) return run_flux_update(c, 1, c->grad);
         return run_flux(c, d, c->qa);
     };
     for (int kind : {int(CND_KIND_FILTER), int(CND_KIND_GRAD), int(CND_KIND_FLUX)}) {
         if (kind == CND_KIND_GRAD && !c->any_visc) continue;
         for (int d = 0; d < 3 && !r; ++d) {
             std::vector<int> cands;
-            if (!mix && !(kind == CND_KIND_FLUX && d == 2 && fz)) cands.push_back(kTiled);   // no fused tiled form
+            if (!mix && !(kind == CND_KIND_FLUX && d == c->fuse_dir && fz)) cands.push_back(kTiled);   // no fused tiled form
             for (int sh = 0; sh < kNumShapes; ++sh)
                 if (shape_sp(c, sh, d, kind_nv(c, kind, d)) > 0) cands.push_back(sh);
             if (cands.size() < 2) continue;

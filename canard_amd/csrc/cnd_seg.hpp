@@ -867,6 +867,23 @@ __device__ __forceinline__ void seg_tile(const BlkDev *__restrict__ blks, const 
                     for (int m = 0; m < NV; ++m) sink.put_raw(g, b, m, sink.fold(fs[kk][m], buf[li[kk] + m * FS]));
                 }
             }
+        } else if constexpr (sink_pre<Sink>::value) {
+            // the sink's own operands of kOutBatch points in flight together
+            static_assert(NV == 1, "prefetching sinks take one variable");
+            for (int e0 = t; e0 < tot; e0 += kOutBatch * nthr) {
+                typename Sink::Pre pr[kOutBatch];
+                int li[kOutBatch];
+#pragma unroll
+                for (int kk = 0; kk < kOutBatch; ++kk) {
+                    const int e = e0 + kk * nthr;
+                    const bool ok = e < elim;
+                    li[kk] = ok ? place(e) : -1;
+                    sink.pre(ok ? g0 + e : g0, pr[kk]);
+                }
+#pragma unroll
+                for (int kk = 0; kk < kOutBatch; ++kk)
+                    if (li[kk] >= 0) sink.put_pre(g0 + (e0 + kk * nthr), buf[li[kk]], pr[kk]);
+            }
         } else {
             for (int e = t; e < elim; e += nthr) {
                 const int li = place(e);

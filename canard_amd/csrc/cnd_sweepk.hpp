@@ -187,25 +187,33 @@ struct SinkFluxV {  // component v of dF/dxi_d; pencil ends to drva
     __device__ void ends(int side, int lane, int, double x) { drva[(side * 5 + v) * FT + fi0 + lane] = x; }
 };
 
-// Component v of dF/dzeta folded straight into the RK update of variable v
+// Component v of the last flux derivative folded straight into the RK update
 // (k_update's arithmetic without the sponge, same operation order, so results
-// are bit-identical): qa = qo - dt/(5-nk) * yaco * ((de_xi + de_eta) + x).
+// are bit-identical): qa = qo - dt/(5-nk) * yaco * ((de_xi + de_eta) + de_zeta),
+// where the sweep's own result x is de_zeta (XFIRST false: da, db the xi and
+// eta buffers) or de_xi (XFIRST true: da, db the eta and zeta buffers).
 // Every operand is variable v at the sink's own point.  Pre holds one point's
 // operands, so sweeps can load them for several points before storing any.
+template <bool XFIRST>
 struct SinkUpdateV {
     double *qa, *drva;
-    const double *qo, *de0, *de1, *met;
+    const double *qo, *da, *db, *met;
     double dtk;   // dt / (5 - nk) of the tile's instance
     int v;
     int64_t NT, FT, fi0;
-    struct Pre { double s, q, r; };
+    struct Pre { double a, b, q, r; };
     __device__ void pre(int64_t g, Pre &o) const
     {
-        o.s = de0[g] + de1[g];
+        o.a = da[g];
+        o.b = db[g];
         o.q = qo[g];
         o.r = dtk * met[9 * NT + g];
     }
-    __device__ void put_pre(int64_t g, double x, const Pre &o) { qa[g] = o.q - o.r * (o.s + x); }
+    __device__ void put_pre(int64_t g, double x, const Pre &o)
+    {
+        const double s = XFIRST ? (x + o.a) + o.b : (o.a + o.b) + x;
+        qa[g] = o.q - o.r * s;
+    }
     __device__ void put(int64_t g, int, int, double x)
     {
         Pre o;
@@ -454,8 +462,8 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
 }
 
 // Last flux sweep of a stage fused with the RK update: sweep of the stored
-// flux component v along d, written as the updated state qa.  Grid x: five
-// workgroups (components) per tile.
+// flux component v along d (xi or zeta), written as the updated state qa.
+// Grid x: five workgroups (components) per tile.
 template <class G, bool MIX>
 __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_update(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
@@ -472,8 +480,9 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
     const Tile t = tiles[tix];
     SrcField<1> src{F + v * NT, 0};
     const double dtk = inst[blks[t.b].inst].dt / double(4 - nk + 1);
-    SinkUpdateV sink{qa + v * NT, drva, qo + v * NT, de + v * NT, de + (5 + v) * NT, met, dtk, v, NT, FT,
-                     blks[t.b].foff[d] + t.q0};
+    // the two other directions' right-hand sides
+    const double *da = de + ((G::XI ? 5 : 0) + v) * NT, *db = de + ((G::XI ? 10 : 5) + v) * NT;
+    SinkUpdateV<G::XI> sink{qa + v * NT, drva, qo + v * NT, da, db, met, dtk, v, NT, FT, blks[t.b].foff[d] + t.q0};
     seg_tile<1, 0, G::L, G::P, G::C, G::XI, MIX, G::SPMAX>(blks, tabs, tb, t, d, 0, SP, src, sink, lds);
 }
 
