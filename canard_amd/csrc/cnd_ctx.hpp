@@ -20,6 +20,8 @@ constexpr int kPts = 256;     // points per pointwise workgroup
 // that workgroups reading or writing whole tiles of points touch whole
 // cache lines of every field; the padding points are never read.
 constexpr int64_t kAlignPts = 256;
+// stored points from which a context runs independent sweeps side by side
+constexpr int64_t kOverlapMinPts = int64_t(4) << 20;
 }  // namespace cnd
 
 // Internal header: every translation unit of libcnd works in namespace cnd.
@@ -51,7 +53,7 @@ struct cnd_ctx {
     // zeta gradients, xi with eta fluxes); ev_fork/ev_join order it with st
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool overlap = true;   // CANARD_OVERLAP=0: every sweep on st
+    bool overlap = true;   // side-by-side sweeps (contexts of >= kOverlapMinPts points; CANARD_OVERLAP)
     BlkDev *dblk = nullptr;
     TabDev *dtab = nullptr;
     InstDev *dinst = nullptr;

@@ -826,19 +826,28 @@ int enqueue_stage(cnd_ctx *c, int nk)
     const double *cur = nk == 1 ? c->qo : c->qa;
     const unsigned ib = unsigned((c->ninst + 63) / 64);
     hipLaunchKernelGGL(k_stage_begin, dim3(ib), dim3(64), 0, c->st, c->dinst, c->ninst, nk);
-    if (nk == 1) {
-        Section sec(c, CND_SEC_TSTEP, 1);
-        hipLaunchKernelGGL(k_tstep, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->met,
-                           c->tmax, c->NT);
-        hipLaunchKernelGGL(k_tstep_reduce, dim3(unsigned(c->ninst)), dim3(kPts), 0, c->st, c->dinst, c->dpt_inst,
-                           int(c->hpt.size()), c->tmax);
-        hipLaunchKernelGGL(k_sched, dim3(ib), dim3(64), 0, c->st, c->dinst, c->dtab, c->ninst);
-    }
     int r;
+    // the time step of stage 1 runs beside the xi gradient sweep of viscous
+    // stages, which reads neither dt nor the maxima (its time then counts in
+    // the gradient section)
+    const bool tstep_side = nk == 1 && c->any_visc && c->overlap;
+    if (nk == 1) {
+        auto tstep = [&] {
+            hipLaunchKernelGGL(k_tstep, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->met,
+                               c->tmax, c->NT);
+            hipLaunchKernelGGL(k_tstep_reduce, dim3(unsigned(c->ninst)), dim3(kPts), 0, c->st, c->dinst,
+                               c->dpt_inst, int(c->hpt.size()), c->tmax);
+            hipLaunchKernelGGL(k_sched, dim3(ib), dim3(64), 0, c->st, c->dinst, c->dtab, c->ninst);
+            return check_launch(c);
+        };
+        Section sec(c, CND_SEC_TSTEP, 1);
+        if ((r = tstep_side ? on_side(c, tstep) : tstep())) return r;
+    }
     if (c->any_visc) {
         // the eta sweep beside the zeta sweep when it is a segmented one
         Section sec(c, CND_SEC_GRAD, 3);
         if ((r = run_grad(c, 0, cur))) return r;
+        if (tstep_side && (r = join_side(c))) return r;
         if (c->overlap && (c->precision == CND_MIXED || seg_on(c, CND_KIND_GRAD, 1))) {
             if ((r = on_side(c, [&] { return run_grad(c, 1, cur); })) || (r = run_grad(c, 2, cur)) ||
                 (r = join_side(c)))
@@ -1069,8 +1078,12 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         c->tune_on = !c->tiled_only && !(tu && std::string(tu) == "0");
         const char *fu = std::getenv("CANARD_FUSE_UPDATE");
         c->fuse_update = !(fu && std::string(fu) == "0");
+        // Side-by-side sweeps pay in a context whose sweeps fill the GPU; in
+        // many small contexts stepped together (each its own stream pair on
+        // a few hardware queues) the cross-stream waits cost far more than
+        // they save.  CANARD_OVERLAP=0 / 1 forces either way.
         const char *ov = std::getenv("CANARD_OVERLAP");
-        c->overlap = !(ov && std::string(ov) == "0");
+        c->overlap = ov ? std::string(ov) != "0" : c->NT >= kOverlapMinPts;
         {
             int dev = 0, lds = 0;
             if (hipGetDevice(&dev) == hipSuccess &&
@@ -1114,9 +1127,11 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         delete c;
         return fail(nullptr, CND_EDEVICE, "stream creation failed");
     }
-    if (hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    // the side stream only where sweeps run side by side: every stream takes
+    // a share of the few hardware queues that many contexts' streams share
+    if (c->overlap && (hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+                       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)) {
         cnd_destroy(c);
         return fail(nullptr, CND_EDEVICE, "side stream creation failed");
     }

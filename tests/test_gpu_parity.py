@@ -326,18 +326,21 @@ def test_stage_by_stage_api_equals_step():
     b.close()
 
 
+@pytest.mark.parametrize("fuse_dir", ["xi", "zeta"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_fused_update_bitwise_equals_separate(monkeypatch, precision):
+def test_fused_update_bitwise_equals_separate(monkeypatch, precision, fuse_dir):
     """Viscous stages without characteristic faces or sponge fold the RK
-    update into the zeta flux sweep; on a periodic TGV (several segments per
-    pencil) the fused step gives bit for bit the state of the separate update
-    kernel (CANARD_FUSE_UPDATE=0) in both precisions, and the FP64 one
-    matches the oracle."""
+    update into the last flux sweep, the xi one (default) or the zeta one
+    (CANARD_FUSE_DIR=2); on a periodic TGV (several segments per pencil) the
+    fused step gives bit for bit the state of the separate update kernel
+    (CANARD_FUSE_UPDATE=0) in both precisions, and the FP64 one matches the
+    oracle."""
     from canard_amd.solver import CND_FP64, CND_MIXED
     blocks, grids, states, params = tgv_case(0, nxyz=(24, 20, 40), pert=1e-3)
     prec = CND_FP64 if precision == "fp64" else CND_MIXED
     monkeypatch.setenv("CANARD_TUNE", "0")
-    print(f"[fused update] {precision}", flush=True)
+    monkeypatch.setenv("CANARD_FUSE_DIR", "0" if fuse_dir == "xi" else "2")
+    print(f"[fused update] {precision} {fuse_dir}", flush=True)
     monkeypatch.setenv("CANARD_FUSE_UPDATE", "0")
     sep = _case(blocks, grids, states, params, precision=prec)
     monkeypatch.setenv("CANARD_FUSE_UPDATE", "1")
@@ -355,3 +358,30 @@ def test_fused_update_bitwise_equals_separate(monkeypatch, precision):
         o.close()
     sep.close()
     fus.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_side_by_side_sweeps_bitwise_equal_serial(monkeypatch, precision):
+    """Independent sweeps on a second stream (the time step beside the xi
+    gradient sweep, eta beside zeta gradients, eta beside the other flux
+    sweep; CANARD_OVERLAP=1, the default from 4 M points) give bit for bit
+    the states of the serial order, eagerly and replayed as a graph."""
+    from canard_amd.solver import CND_FP64, CND_MIXED
+    blocks, grids, states, params = tgv_case(0, nxyz=(32, 28, 24), pert=1e-3)
+    prec = CND_FP64 if precision == "fp64" else CND_MIXED
+    print(f"[side by side] {precision}", flush=True)
+    monkeypatch.setenv("CANARD_TUNE", "0")      # the same (default) shapes in all three
+    monkeypatch.setenv("CANARD_OVERLAP", "0")
+    ser = _case(blocks, grids, states, params, precision=prec)
+    monkeypatch.setenv("CANARD_OVERLAP", "1")
+    par = _case(blocks, grids, states, params, precision=prec)
+    gra = _case(blocks, grids, states, params, precision=prec)
+    gra.use_graph(True)
+    for k in range(3):
+        for c in (ser, par, gra):
+            c.step(4)
+        assert np.array_equal(ser.state(0), par.state(0)), k
+        assert np.array_equal(ser.state(0), gra.state(0)), k
+    assert ser.clock() == par.clock() == gra.clock()
+    for c in (ser, par, gra):
+        c.close()
