@@ -506,30 +506,47 @@ __global__ __launch_bounds__(kLine) void k_wall(const BlkDev *__restrict__ blks,
 // as one gather and one scatter over a JunctionProgram: an edge entry reads a
 // point that corner averaging changes through that corner's entry, which is
 // recomputed in the same order of additions.
-__device__ __forceinline__ double junc_corner(const JuncDev &J, const double *q, int j, int m, int64_t NT)
+__device__ __forceinline__ double junc_corner(const JuncDev &J, const double *q, int j)
 {
-    double acc = q[m * NT + J.tgt[j]];
-    for (int k = J.first[j]; k < J.first[j + 1]; ++k) acc = acc + q[m * NT + J.src[k]];
+    double acc = q[J.tgt[j]];
+    for (int k = J.first[j]; k < J.first[j + 1]; ++k) acc = acc + q[J.src[k]];
     return J.scale[j] * acc;
 }
 
+// value of source k of an entry (variable plane q): the point itself, or a
+// corner's average
+__device__ __forceinline__ double junc_src(const JuncDev &J, const double *q, int k)
+{
+    const int c = J.sc[k];
+    return c < 0 ? q[J.src[k]] : junc_corner(J, q, c);
+}
+
+// One thread per (entry, variable = blockIdx.y).  The sources are loaded a
+// chunk at a time before they are added, in order, so their loads overlap.
 __global__ void k_junc_gather(const JuncDev J, const double *__restrict__ q, int64_t NT)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int kChunk = 4;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
     if (i >= J.n || !J.write[i]) return;
-    for (int m = 0; m < 5; ++m) {
-        double acc = J.tc[i] < 0 ? q[m * NT + J.tgt[i]] : junc_corner(J, q, J.tc[i], m, NT);
-        for (int k = J.first[i]; k < J.first[i + 1]; ++k)
-            acc = acc + (J.sc[k] < 0 ? q[m * NT + J.src[k]] : junc_corner(J, q, J.sc[k], m, NT));
-        J.tmp[m * int64_t(J.n) + i] = J.scale[i] * acc;
+    const double *qm = q + m * NT;
+    const int k0 = J.first[i], k1 = J.first[i + 1];
+    double acc = J.tc[i] < 0 ? qm[J.tgt[i]] : junc_corner(J, qm, J.tc[i]);
+    for (int kb = k0; kb < k1; kb += kChunk) {
+        double v[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) v[j] = kb + j < k1 ? junc_src(J, qm, kb + j) : 0.0;
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j)
+            if (kb + j < k1) acc = acc + v[j];
     }
+    J.tmp[m * int64_t(J.n) + i] = J.scale[i] * acc;
 }
 
 __global__ void k_junc_scatter(const JuncDev J, double *q, int64_t NT)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
     if (i >= J.n || !J.write[i]) return;
-    for (int m = 0; m < 5; ++m) q[m * NT + J.tgt[i]] = J.tmp[m * int64_t(J.n) + i];
+    q[m * NT + J.tgt[i]] = J.tmp[m * int64_t(J.n) + i];
 }
 
 // The face tiles of all three directions in one launch: blockIdx.x runs over
@@ -913,8 +930,8 @@ int enqueue_stage(cnd_ctx *c, int nk)
                                    c->dtiles[d], d, s, c->qa, c->NT);
     if (c->jn.n > 0) {
         const unsigned nb = unsigned((c->jn.n + 127) / 128);
-        hipLaunchKernelGGL(k_junc_gather, dim3(nb), dim3(128), 0, c->st, c->jn, c->qa, c->NT);
-        hipLaunchKernelGGL(k_junc_scatter, dim3(nb), dim3(128), 0, c->st, c->jn, c->qa, c->NT);
+        hipLaunchKernelGGL(k_junc_gather, dim3(nb, 5), dim3(128), 0, c->st, c->jn, c->qa, c->NT);
+        hipLaunchKernelGGL(k_junc_scatter, dim3(nb, 5), dim3(128), 0, c->st, c->jn, c->qa, c->NT);
     }
     if (c->any_avg) {
         FaceSet fs;
