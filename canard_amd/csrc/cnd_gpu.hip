@@ -610,8 +610,11 @@ __global__ __launch_bounds__(kLine) void k_surf_avg(const BlkDev *__restrict__ b
     idx[2] = int(lloc / (int64_t(B.n[0]) * B.n[1]));
     for (int dd = d + 1; dd < 3; ++dd)
         if (on_avg_face(B, dd, 0, idx) || on_avg_face(B, dd, 1, idx)) return;
+    // the point's value: its own face's snapshot (nothing writes q between the
+    // two launches), read coalesced where q's face points are a pencil apart
+    const double *own = fs.sv[d] + side * 5 * fs.FT[d] + B.foff[d] + qq;
     double v[5];
-    for (int m = 0; m < 5; ++m) v[m] = q[m * NT + g];
+    for (int m = 0; m < 5; ++m) v[m] = own[m * fs.FT[d]];
     double cnt = 1.0;
     for (int f = 0; f <= 2 * d + side; ++f) {
         const int dd = f / 2, s = f % 2;
