@@ -358,11 +358,12 @@ using Sh16 = SegG<17, 24, 16, 16, true>;
 // workgroup size bound of a kernel (shapes whose NV * P * SPMAX exceeds the
 // 1024-thread limit are rejected at launch time by shape_sp)
 __host__ __device__ constexpr int wg_bound(int x) { return x > 1024 ? 1024 : x; }
-// waves per SIMD a segmented sweep is compiled for: xi shapes of 17-point
-// segments are held to 128 VGPRs (4 waves), which their LDS also allows;
-// the others are left to the compiler
-template <class G>
-__host__ __device__ constexpr int seg_waves() { return G::XI && G::L <= 17 ? 4 : 1; }
+// waves per SIMD a segmented sweep is compiled for: shapes of 17-point
+// segments are held to 128 VGPRs (4 waves, so two 512-thread workgroups
+// share a CU), which their LDS also allows -- eta/zeta ones in FP64 only,
+// where it costs no spills; the others are left to the compiler
+template <class G, bool MIX>
+__host__ __device__ constexpr int seg_waves() { return G::L <= 17 && (G::XI || !MIX) ? 4 : 1; }
 constexpr int kTiled = -1;   // shape index of the tiled sweep
 // pencils-per-workgroup values (tile lists) and segment lengths (tables)
 constexpr int kNumP = 7, kPs[kNumP] = {4, 8, 16, 32, 64, 20, 24};
@@ -379,7 +380,7 @@ __host__ __device__ constexpr int l_index(int L) { return L == 17 ? 0 : 1; }
     extern __shared__ double lds[];
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_fd_field(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), (seg_waves<G, MIX>())) void k_seg_fd_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, int n45, const double *in, int64_t in_vstride, double *out, int64_t out_vstride)
 {
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
 }
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_filter_field(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), (seg_waves<G, MIX>())) void k_seg_filter_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *in, double *out, int64_t vstride)
 {
@@ -406,7 +407,7 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
 // in FP32.  prim (xi sweeps, or null): the primitives u, v, w, T are also
 // written there.
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX), seg_waves<G>()) void k_seg_grad(
+__global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX), (seg_waves<G, MIX>())) void k_seg_grad(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *q, typename std::conditional<MIX, float, double>::type *grad,
     int64_t NT, double *prim)
@@ -422,7 +423,7 @@ __global__ __launch_bounds__(wg_bound(4 * G::P * G::SPMAX), seg_waves<G>()) void
 // gradient sweep stored (prim, four fields NT apart), into grad; in mixed
 // mode FP32.
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_grad_prim(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), (seg_waves<G, MIX>())) void k_seg_grad_prim(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *prim, typename std::conditional<MIX, float, double>::type *grad,
     int64_t NT)
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
 }
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux(
+__global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX), (seg_waves<G, MIX>())) void k_seg_flux(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *q, const double *met, double *de,
     double *drva, int64_t NT, int64_t FT)
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(wg_bound(5 * G::P * G::SPMAX), seg_waves<G>()) void
 }
 
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_field(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), (seg_waves<G, MIX>())) void k_seg_flux_field(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const double *F, double *de, double *drva, int64_t NT, int64_t FT)
 {
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_s
 // flux component v along d (xi or zeta), written as the updated state qa.
 // Grid x: five workgroups (components) per tile.
 template <class G, bool MIX>
-__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), seg_waves<G>()) void k_seg_flux_update(
+__global__ __launch_bounds__(wg_bound(G::P * G::SPMAX), (seg_waves<G, MIX>())) void k_seg_flux_update(
     const BlkDev *__restrict__ blks, const TabDev *__restrict__ tabs, SegTabs tb, const Tile *__restrict__ tiles,
     int ntiles, int SP, int d, const InstDev *__restrict__ inst, const double *F, const double *de,
     const double *__restrict__ qo, const double *__restrict__ met, double *qa, double *drva, int64_t NT, int64_t FT,
