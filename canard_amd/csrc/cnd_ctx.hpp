@@ -78,12 +78,11 @@ struct cnd_ctx {
     double tune_ms[CND_NKIND][3] = {};
     std::vector<Tile> hpt;
     Tile *dpt = nullptr;
-    int *dpt_inst = nullptr;   // the instance of each point tile
     // de: right-hand side as three 5-field buffers, one per flux direction
     // (summed in the update); fl: the 15 stored fluxes F[5 d + m] of viscous
     // stages, scratch for metrics, single sweeps and the autotuner otherwise
     double *qa = nullptr, *qo = nullptr, *de = nullptr, *grad = nullptr, *fl = nullptr, *met = nullptr;
-    double *ys = nullptr, *xyz = nullptr, *sponge = nullptr, *sp_par = nullptr, *tmax = nullptr;
+    double *ys = nullptr, *xyz = nullptr, *sponge = nullptr, *sp_par = nullptr;
     int nmax[3] = {0, 0, 0};   // longest pencil per direction
     bool part_other[3] = {};   // some block has another block as interface partner along d
     int64_t ys_len = 0;        // doubles in the forward-value scratch

@@ -94,15 +94,19 @@ __device__ __forceinline__ double wave_max(double v)
 
 // Spectral-radius maxima of calc_time_step (src/mo_physics.f90:185-207);
 // runs only on steps where mod(n,10) == 1.  Each workgroup (one point tile)
-// writes its two maxima; k_tstep_reduce folds them per instance.
-__global__ __launch_bounds__(kPts) void k_tstep(const BlkDev *__restrict__ blks, const InstDev *__restrict__ inst,
+// folds its two maxima into its instance's (I.red, cleared at stage 1) with
+// an atomic max of their bit patterns, which order like the values for
+// non-negative doubles: the result does not depend on the order of the
+// workgroups.  A workgroup whose maximum does not exceed what it reads
+// there already skips the atomic.
+__global__ __launch_bounds__(kPts) void k_tstep(const BlkDev *__restrict__ blks, InstDev *__restrict__ inst,
                                                const Tile *__restrict__ tiles, const double *__restrict__ q,
-                                               const double *__restrict__ met, double *__restrict__ tmax, int64_t NT)
+                                               const double *__restrict__ met, int64_t NT)
 {
     __shared__ double red[2][kPts / 64];
     const Tile t = tiles[blockIdx.x];
     const BlkDev &B = blks[t.b];
-    const InstDev &I = inst[B.inst];
+    InstDev &I = inst[B.inst];
     if (I.n % 10 != 1) return;
     const int64_t npt = int64_t(B.n[0]) * B.n[1] * B.n[2];
     const int64_t lp = int64_t(t.q0) + threadIdx.x;
@@ -133,34 +137,11 @@ __global__ __launch_bounds__(kPts) void k_tstep(const BlkDev *__restrict__ blks,
     if (threadIdx.x == 0) {
         double a = red[0][0], c = red[1][0];
         for (int k = 1; k < kPts / 64; ++k) { a = fmax(a, red[0][k]); c = fmax(c, red[1][k]); }
-        tmax[2 * int64_t(blockIdx.x)] = a;
-        tmax[2 * int64_t(blockIdx.x) + 1] = c;
-    }
-}
-
-// per-instance maximum over the point tiles (one workgroup per instance;
-// tinst: the instance of each tile, so a tile costs independent loads only)
-__global__ __launch_bounds__(kPts) void k_tstep_reduce(InstDev *__restrict__ inst, const int *__restrict__ tinst,
-                                                      int ntiles, const double *__restrict__ tmax)
-{
-    __shared__ double red[2][kPts / 64];
-    InstDev &I = inst[blockIdx.x];
-    if (I.n % 10 != 1) return;
-    double a = 0.0, c = 0.0;
-#pragma unroll 4
-    for (int k = threadIdx.x; k < ntiles; k += kPts)
-        if (tinst[k] == int(blockIdx.x)) {
-            a = fmax(a, tmax[2 * int64_t(k)]);
-            c = fmax(c, tmax[2 * int64_t(k) + 1]);
-        }
-    a = wave_max(a);
-    c = wave_max(c);
-    if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x / 64] = a; red[1][threadIdx.x / 64] = c; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < kPts / 64; ++k) { red[0][0] = fmax(red[0][0], red[0][k]); red[1][0] = fmax(red[1][0], red[1][k]); }
-        I.red[0] = (unsigned long long)__double_as_longlong(red[0][0]);
-        I.red[1] = (unsigned long long)__double_as_longlong(red[1][0]);
+        const unsigned long long ab = (unsigned long long)__double_as_longlong(a),
+                                 cb = (unsigned long long)__double_as_longlong(c);
+        volatile unsigned long long *seen = I.red;
+        if (a >= 0.0 && ab > seen[0]) atomicMax(&I.red[0], ab);
+        if (c >= 0.0 && cb > seen[1]) atomicMax(&I.red[1], cb);
     }
 }
 
@@ -854,9 +835,7 @@ int enqueue_stage(cnd_ctx *c, int nk)
     if (nk == 1) {
         auto tstep = [&] {
             hipLaunchKernelGGL(k_tstep, pts_grid(c), dim3(kPts), 0, c->st, c->dblk, c->dinst, c->dpt, cur, c->met,
-                               c->tmax, c->NT);
-            hipLaunchKernelGGL(k_tstep_reduce, dim3(unsigned(c->ninst)), dim3(kPts), 0, c->st, c->dinst,
-                               c->dpt_inst, int(c->hpt.size()), c->tmax);
+                               c->NT);
             hipLaunchKernelGGL(k_sched, dim3(ib), dim3(64), 0, c->st, c->dinst, c->dtab, c->ninst);
             return check_launch(c);
         };
@@ -1174,8 +1153,7 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
     for (int d = 0; good && d < 3; ++d)
         for (int pi = 0; good && pi < kNumP; ++pi) good = ok(dalloc(c, &c->dstiles[pi][d], c->hstiles[pi][d].size()));
     if (good)
-        good = ok(dalloc(c, &c->dpt, c->hpt.size())) && ok(dalloc(c, &c->dpt_inst, c->hpt.size())) &&
-               ok(dalloc(c, &c->tmax, 2 * c->hpt.size()));
+        good = ok(dalloc(c, &c->dpt, c->hpt.size()));
     if (!good) { std::string m = c->err; cnd_destroy(c); return fail(nullptr, r ? r : CND_ENOMEM, m); }
     std::vector<TabDev> ht(ninst);
     for (int i = 0; i < ninst; ++i) {
@@ -1218,11 +1196,6 @@ int cnd_create(const cnd_params *params, int ninst, const cnd_block *blocks, int
         up(c->dlhs, lhs.data(), sizeof(double) * lhs.size());
     }
     up(c->dpt, c->hpt.data(), sizeof(Tile) * c->hpt.size());
-    {
-        std::vector<int> ti(c->hpt.size());
-        for (size_t k = 0; k < ti.size(); ++k) ti[k] = c->spec[c->hpt[k].b].inst;
-        up(c->dpt_inst, ti.data(), sizeof(int) * ti.size());
-    }
     if (!r) r = upload_inst(c);
     if (!r) {
         hipError_t e = hipMemset(c->de, 0, sizeof(double) * 15 * NT);
@@ -1266,7 +1239,7 @@ void cnd_destroy(cnd_ctx *c)
     for (auto &m : c->marks) { hipEventDestroy(m.a); hipEventDestroy(m.b); }
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     void *ptrs[] = {c->dblk, c->dtab, c->dinst, c->dlu, c->qa, c->qo, c->de, c->grad, c->fl, c->met, c->ys,
-                    c->xyz, c->sponge, c->sp_par, c->tmax, c->dpt, c->dpt_inst, c->gradf, c->dlu32, c->dlhs, c->jn.tgt, c->jn.src,
+                    c->xyz, c->sponge, c->sp_par, c->dpt, c->gradf, c->dlu32, c->dlhs, c->jn.tgt, c->jn.src,
                     c->jn.first, c->jn.tc, c->jn.sc, c->jn.write, c->jn.scale, c->jn.tmp, c->qb};
     for (void *p : ptrs) if (p) hipFree(p);
     for (int li = 0; li < kNumL; ++li) {
