@@ -44,9 +44,9 @@ STEP_BYTES_MODEL = 5048.0  # compulsory FP64 bytes per stored point per viscous 
 #                               right-hand-side buffer (5)
 #   update: read qo 5, the three right-hand-side buffers (15), Jacobian;
 #           write qa 5.  Viscous stages without characteristic faces run it
-#           as the zeta flux sweep (the "flux" group then has two launches):
-#           read the 5 zeta fluxes, the xi and eta buffers (10), qo 5,
-#           Jacobian; write qa 5 -- the same 208 B
+#           in the xi flux sweep (the "flux" group then has the eta and zeta
+#           launches): read the 5 xi fluxes, the eta and zeta buffers (10),
+#           qo 5, Jacobian; write qa 5 -- the same 208 B
 SECTION_BYTES = {"filter": 80.0, "grad": 232.0 / 3.0, "stress": 336.0, "flux": 80.0, "update": 208.0}
 
 

@@ -240,8 +240,9 @@ int run_grad(cnd_ctx *c, int d, const double *cur);
 // Flux sweep along d into the d-th right-hand-side buffer (kind
 // CND_KIND_FLUX).
 int run_flux(cnd_ctx *c, int d, const double *cur);
-// zeta flux sweep of the stored fluxes fused with the RK update of stage nk,
-// written to qa.
+// Flux sweep of the stored fluxes along fuse_dir (xi, or zeta) fused with the
+// RK update of stage nk, written to qa; the other two directions' buffers
+// must be complete.
 int run_flux_update(cnd_ctx *c, int nk, double *qa);
 
 // Joins an output write in flight and frees the output buffers (cnd_destroy).
