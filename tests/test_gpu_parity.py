@@ -360,16 +360,33 @@ def test_fused_update_bitwise_equals_separate(monkeypatch, precision, fuse_dir):
     fus.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_side_by_side_sweeps_bitwise_equal_serial(monkeypatch, precision):
+def _side_by_side_case(case):
+    if case == "tgv":
+        return tgv_case(0, nxyz=(32, 28, 24), pert=1e-3)
+    sizes = [(24, 16, 14), (16, 16, 14)]
+    blocks = cases.sheared_pair_blocks(sizes)
+    grids = cases.sheared_pair_grids(sizes, h=1.0, amp=0.05)
+    states = [cases.pulse_state(g, mach=0.5, centres=[(20.0, 8.0, 7.0)], amp=1e-3, width=3.0) for g in grids]
+    params = dict(reoo=200.0, amach1=0.5, nsmf=1, cfl=0.8, sponge=1, szco=0.5, szth0=6.0, szth1=6.0,
+                  doml0=0.0, doml1=40.0, viscous=0 if case == "gcbc_inviscid" else 1)
+    return blocks, grids, states, params
+
+
+@pytest.mark.parametrize("precision,case", [("fp64", "tgv"), ("mixed", "tgv"), ("fp64", "gcbc_sponge"),
+                                            ("fp64", "gcbc_inviscid")])
+def test_side_by_side_sweeps_bitwise_equal_serial(monkeypatch, precision, case):
     """Independent sweeps on a second stream (the time step beside the xi
     gradient sweep, eta beside zeta gradients, eta beside the other flux
-    sweep; CANARD_OVERLAP=1, the default from 4 M points) give bit for bit
-    the states of the serial order, eagerly and replayed as a graph."""
+    sweeps; CANARD_OVERLAP=1, the default from 4 M points) give bit for bit
+    the states of the serial order, eagerly and replayed as a graph: on a
+    periodic TGV (update fused into the xi flux sweep), on the sheared
+    two-block case with characteristic faces and sponge (all three flux
+    sweeps, then GCBC and the separate update), and on that case inviscid
+    (fluxes evaluated in the sweeps)."""
     from canard_amd.solver import CND_FP64, CND_MIXED
-    blocks, grids, states, params = tgv_case(0, nxyz=(32, 28, 24), pert=1e-3)
+    blocks, grids, states, params = _side_by_side_case(case)
     prec = CND_FP64 if precision == "fp64" else CND_MIXED
-    print(f"[side by side] {precision}", flush=True)
+    print(f"[side by side] {precision} {case}", flush=True)
     monkeypatch.setenv("CANARD_TUNE", "0")      # the same (default) shapes in all three
     monkeypatch.setenv("CANARD_OVERLAP", "0")
     ser = _case(blocks, grids, states, params, precision=prec)
@@ -380,8 +397,9 @@ def test_side_by_side_sweeps_bitwise_equal_serial(monkeypatch, precision):
     for k in range(3):
         for c in (ser, par, gra):
             c.step(4)
-        assert np.array_equal(ser.state(0), par.state(0)), k
-        assert np.array_equal(ser.state(0), gra.state(0)), k
+        for b in range(len(blocks)):
+            assert np.array_equal(ser.state(b), par.state(b)), (k, b)
+            assert np.array_equal(ser.state(b), gra.state(b)), (k, b)
     assert ser.clock() == par.clock() == gra.clock()
     for c in (ser, par, gra):
         c.close()
